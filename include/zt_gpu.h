@@ -1,0 +1,129 @@
+/* zt_gpu.h -- the GPU boundary of zigtrace's path-tracing core (gfx950 / MI355X).
+ *
+ * This is the C ABI a host (the reference's Zig `render.render`, or the C++
+ * host in zt_host.h) calls instead of running the CPU tile loop. Everything
+ * below `render.zig:152` -- camera ray generation, BVH traversal, shading,
+ * next-event estimation with MIS, Russian roulette and the per-pixel
+ * accumulation -- runs on the device. Scene construction (parse, Bvh.build,
+ * Lights.build) stays on the host and is handed over as a flat view.
+ *
+ * Conventions
+ *   - Every function returns an int status from zt_types.h (ZT_OK == 0) unless
+ *     stated otherwise. No C++ exception crosses this boundary.
+ *   - Calls are blocking and synchronous on the context's HIP stream. A
+ *     context is not re-entrant: use one per host thread.
+ *   - Host buffers passed in are copied; the caller keeps ownership.
+ */
+#ifndef ZT_GPU_H
+#define ZT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "zt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zt_ctx zt_ctx;
+
+/* A built scene (scene.zig:39-52 after `Scene.build`, scene.zig:92-96)
+ * flattened into plain arrays. `shapes` are the bounded shapes in
+ * `Scene.shapes` order, `planes` the unbounded ones (scene.zig:71-78).
+ * `nodes`/`indices` are the host-built `Bvh` (bvh.zig:35-38); num_nodes == 0
+ * with num_shapes > 0 means "no BVH" and selects the brute-force path of
+ * scene.zig:109-112. `lights`/`light_cdf`/`light_total` are `Lights`
+ * (light.zig:59-63). */
+typedef struct zt_scene_view {
+  const zt_material* materials;
+  uint64_t num_materials;
+  const zt_shape* shapes;
+  uint64_t num_shapes;
+  const zt_shape* planes;
+  uint64_t num_planes;
+  const zt_bvh_node* nodes;
+  uint64_t num_nodes;
+  const uint32_t* indices; /* num_shapes entries when num_nodes > 0 */
+  const zt_light* lights;
+  uint64_t num_lights;
+  const double* light_cdf; /* num_lights entries */
+  double light_total;
+  zt_camera_config camera;
+  zt_background background;
+} zt_scene_view;
+
+/* GPU-only knobs. None of them changes the image: the same seed gives the
+ * same bits for every value (tested). Zero means "default / tuned". */
+enum { ZT_METHOD_AUTO = 0, ZT_METHOD_MEGAKERNEL = 1, ZT_METHOD_WAVEFRONT = 2 };
+enum { ZT_PRECISION_FP32 = 0, ZT_PRECISION_FP64 = 1 };
+typedef struct zt_gpu_tuning {
+  uint32_t method;      /* ZT_METHOD_* */
+  uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
+  uint32_t tile_w;      /* megakernel: pixels per workgroup row (0 = default) */
+  uint32_t tile_h;      /* megakernel: pixel rows per workgroup (0 = default) */
+  uint32_t block_size;  /* wavefront: threads per workgroup (0 = default) */
+  uint32_t queue_paths; /* wavefront: paths in flight per pass (0 = every pixel) */
+  uint32_t _reserved[2];
+} zt_gpu_tuning;
+
+/* Device-side timing and integer work counters of the last render call. */
+typedef struct zt_gpu_stats {
+  double kernel_ms;    /* HIP-event time of the render's kernels on the context stream */
+  double upload_ms;    /* host wall time of the last scene upload */
+  uint64_t launches;   /* kernel launches in the last render */
+  uint64_t paths;      /* camera samples traced */
+  uint64_t extend_rays;
+  uint64_t shadow_rays;
+} zt_gpu_stats;
+
+/* Context ---------------------------------------------------------------- */
+/* Owns a HIP stream on `device`, the uploaded scenes and scratch buffers.
+ * Replaces the thread pool created at render.zig:169-177. */
+int zt_gpu_create(int device, zt_ctx** out);
+void zt_gpu_destroy(zt_ctx* ctx);
+/* Message for the last failing call on this context (never NULL). */
+const char* zt_gpu_last_error(const zt_ctx* ctx);
+/* The context's HIP stream (a hipStream_t), for callers that time or order work. */
+void* zt_gpu_stream(zt_ctx* ctx);
+int zt_gpu_get_stats(const zt_ctx* ctx, zt_gpu_stats* out);
+
+/* Scenes ----------------------------------------------------------------- */
+/* Copies a built scene to the device in the GPU layout (64-byte two-child
+ * BVH nodes, leaf-ordered primitives, fp32 and on request fp64 copies).
+ * Validates every index; returns ZT_ERR_BAD_SCENE on inconsistency. */
+int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* view, uint32_t* scene_id);
+int zt_gpu_scene_release(zt_ctx* ctx, uint32_t scene_id);
+
+/* Rendering -------------------------------------------------------------- */
+/* render.zig:152-180. Validation matches render.zig:153 (width, height, spp
+ * or tile_size == 0 -> ZT_ERR_INVALID_OPTIONS, checked first). Writes
+ * width*height*3 values, row-major y*width+x, RGB: float for
+ * ZT_PRECISION_FP32, double for ZT_PRECISION_FP64. `out_on_device` != 0 means
+ * `out` is device memory (no host copy). `tuning` may be NULL. */
+int zt_gpu_render(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
+                  void* out, int out_on_device);
+
+/* traceWith (render.zig:61-102) for n caller-given rays, each with its own
+ * xoshiro256++ state (4 x u64 per ray, advanced in place as the reference
+ * advances `*Rng`). Radiance goes to out_rgb (n*3 doubles). Host buffers. */
+int zt_gpu_trace_rays(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, uint64_t* rng_states,
+                      uint64_t max_depth, uint32_t light_sampling, uint32_t precision, double* out_rgb);
+
+/* Scene.intersect (scene.zig:108-116) for n rays: mode 0 = BVH + planes,
+ * mode 1 = brute force over every primitive (geometry.zig:129-139). */
+int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, double t_min, double t_max,
+                     int mode, uint32_t precision, zt_hit* out);
+
+/* Rng.forStream(seed, streams[i]) followed by `draws` calls of next()
+ * (rng.zig:22-38) per stream: out[i*draws + k]. kind 0 = next() bits,
+ * kind 1 = the fp32 uniform the device path uses, stored as its f32 bit
+ * pattern in the low 32 bits. */
+int zt_gpu_rng(zt_ctx* ctx, uint64_t seed, const uint64_t* streams, uint64_t n, uint32_t draws, int kind,
+               uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZT_GPU_H */

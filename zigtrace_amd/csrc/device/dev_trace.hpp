@@ -1,0 +1,337 @@
+// Ray/primitive tests and the nearest-child-first BVH traversal.
+//
+// Semantics follow geometry.zig:37-99 (open interval (t_min, t_max), near
+// root before far root, Moller-Trumbore with |det| < 1e-12 rejection) and
+// bvh.zig:80-126 (prune popped entries whose entry distance exceeds the
+// closest hit; push the farther child first; ties go left).
+//
+// Two arithmetic regimes:
+//   * double: the reference's f64 formulas operation for operation. BVH
+//     boxes are the fp32 boxes rounded outward, so traversal visits a
+//     superset of the reference's nodes and the closest hit is unchanged.
+//   * float: the product path. Self-intersection is avoided by identity, not
+//     by an offset: a ray leaving a flat primitive never tests it again, and
+//     a ray leaving a sphere only finds that sphere's far side (t = 2h/a).
+//     The reference's t_min = 1e-4 still applies to every other primitive.
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace ztd {
+
+// Closest hit (or any hit) result: t and the primitive id, where ids
+// [0, num_prims) are leaf-ordered BVH primitives and num_prims + j is plane j.
+template <typename R>
+struct HitRec {
+  R t;
+  int prim;  // -1: no hit
+};
+
+template <typename R>
+ZT_D bool sphereHit(const Prim<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout) {
+  const R radius = p.b.x;
+  if constexpr (K<R>::exact) {  // geometry.zig:37-49
+    const V3<R> oc = sub(p.a, o);
+    const R a = len2(d);
+    const R h = dot(d, oc);
+    const R c = len2(oc) - radius * radius;
+    const R disc = h * h - a * c;
+    if (disc < 0) return false;
+    const R sq = sqrt(disc);
+    R t = (h - sq) / a;
+    if (t <= tmin || t >= tmax) {
+      t = (h + sq) / a;
+      if (t <= tmin || t >= tmax) return false;
+    }
+    *tout = t;
+    return true;
+  } else {
+    // Same roots, computed with the perpendicular-distance discriminant and
+    // the stable quadratic form (Haines et al., Ray Tracing Gems ch. 7).
+    const V3<R> f = sub(o, p.a);
+    const R a = len2(d);
+    const R h = -dot(f, d);
+    const V3<R> l = add(f, scale(d, h / a));
+    const R disc = a * (radius * radius - len2(l));
+    if (disc < 0) return false;
+    const R c = len2(f) - radius * radius;
+    const R q = h + copysign(sqrt(disc), h);
+    const R t0 = c / q, t1 = q / a;
+    const R tn = fmin(t0, t1), tf = fmax(t0, t1);
+    R t = tn;
+    if (!(t > tmin && t < tmax)) {
+      t = tf;
+      if (!(t > tmin && t < tmax)) return false;
+    }
+    *tout = t;
+    return true;
+  }
+}
+
+// A ray that starts on sphere `p`: its only possible hit on that sphere is
+// the far end of the chord, t = 2h/a (the near root is the origin itself).
+template <typename R>
+ZT_D bool selfSphereHit(const Prim<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout) {
+  const R h = dot(d, sub(p.a, o));
+  const R t = R(2) * h / len2(d);
+  if (!(t > tmin && t < tmax)) return false;
+  *tout = t;
+  return true;
+}
+
+template <typename R>
+ZT_D bool triangleHit(const Prim<R>& p, V3<R> o, V3<R> d, R tmin, R tmax, R* tout) {  // geometry.zig:83-97
+  const V3<R> e1 = p.b, e2 = p.c;
+  const V3<R> pv = cross(d, e2);
+  const R det = dot(e1, pv);
+  if (fabs(det) < R(1e-12)) return false;
+  const R inv_det = R(1) / det;
+  const V3<R> s = sub(o, p.a);
+  const R u = dot(s, pv) * inv_det;
+  if (u < 0 || u > 1) return false;
+  const V3<R> q = cross(s, e1);
+  const R v = dot(d, q) * inv_det;
+  if (v < 0 || u + v > 1) return false;
+  const R t = dot(e2, q) * inv_det;
+  if (t <= tmin || t >= tmax) return false;
+  *tout = t;
+  return true;
+}
+
+template <typename R>
+ZT_D bool planeHit(const Plane<R>& pl, V3<R> o, V3<R> d, R tmin, R tmax, R* tout) {  // geometry.zig:67-73
+  const R denom = dot(pl.normal, d);
+  if (fabs(denom) < R(1e-12)) return false;
+  const R t = dot(sub(pl.point, o), pl.normal) / denom;
+  if (t <= tmin || t >= tmax) return false;
+  *tout = t;
+  return true;
+}
+
+// Slab test of one child box (aabb.zig:68-84): lo starts at t_min, hi at the
+// current closest hit; a NaN slab (0*inf) leaves the interval unchanged,
+// which fmax/fmin reproduce. The fp32 path widens the far distance by
+// 1 + 2*gamma(3) so rounding never loses a box the exact test would hit.
+template <typename R>
+ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1, V3<R> o, V3<R> inv, bool nx,
+               bool ny, bool nz, R tmin, R tmax, R* tout) {
+  const R tx0 = (R(nx ? bx1 : bx0) - o.x) * inv.x;
+  const R tx1 = (R(nx ? bx0 : bx1) - o.x) * inv.x;
+  const R ty0 = (R(ny ? by1 : by0) - o.y) * inv.y;
+  const R ty1 = (R(ny ? by0 : by1) - o.y) * inv.y;
+  const R tz0 = (R(nz ? bz1 : bz0) - o.z) * inv.z;
+  const R tz1 = (R(nz ? bz0 : bz1) - o.z) * inv.z;
+  const R lo = fmax(fmax(tmin, tx0), fmax(ty0, tz0));
+  R far = fmin(tx1, fmin(ty1, tz1));
+  if constexpr (!K<R>::exact) far = far * (1.0f + 3.0f * 0x1.0p-23f);
+  const R hi = fmin(tmax, far);
+  *tout = lo;
+  return !(hi < lo);
+}
+
+// Per-lane traversal stack: the top kLds entries live in LDS (laid out
+// [entry][lane], 4-byte words, so a wave's accesses are bank-conflict free
+// whatever its lanes' depths), the rest spill in halves to private memory.
+static constexpr int kLds = 16;
+static constexpr int kHalf = kLds / 2;
+static constexpr int kSpill = 64;
+// Deepest device tree the stack can traverse (checked at upload): a BVH
+// needs at most depth + 1 entries with nearest-first pushes.
+static constexpr int kMaxTreeDepth = kLds + kSpill - 1;
+
+struct LaneStack {
+  int* code;    // LDS: code[i * stride]
+  float* tval;  // LDS: tval[i * stride]
+  int stride;
+};
+
+template <typename R>
+ZT_D float entryT(R t) {
+  if constexpr (K<R>::exact) return __double2float_rd(t);  // rounding down keeps pruning conservative
+  else return t;
+}
+
+// Nearest-hit (anyhit == false) or any-hit query over BVH + planes.
+// self/excl: primitive ids to treat specially (fp32 only, see above); -1 none.
+// Sets *overflow if the stack would exceed its capacity (cannot happen for a
+// host BVH of depth <= 66, kept as a checked invariant).
+template <typename R>
+ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V3<R> d, R tmin, R tmax, int self,
+                        int excl, bool anyhit, int* overflow) {
+  constexpr bool kIdentity = !K<R>::exact;
+  R closest = tmax;
+  int best = -1;
+  bool done = false;
+  if (sc.num_nodes > 0) {
+    const V3<R> inv = mk(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    int spill_code[kSpill];
+    float spill_t[kSpill];
+    int sp = 0, nsp = 0;
+    stk.code[0] = 0;
+    stk.tval[0] = entryT<R>(tmin);
+    sp = 1;
+    auto push = [&](int c, R t) {
+      if (sp == kLds) {
+        if (nsp + kHalf > kSpill) {
+          *overflow = 1;
+          return;
+        }
+        for (int i = 0; i < kHalf; ++i) {
+          spill_code[nsp + i] = stk.code[i * stk.stride];
+          spill_t[nsp + i] = stk.tval[i * stk.stride];
+          stk.code[i * stk.stride] = stk.code[(i + kHalf) * stk.stride];
+          stk.tval[i * stk.stride] = stk.tval[(i + kHalf) * stk.stride];
+        }
+        nsp += kHalf;
+        sp = kHalf;
+      }
+      stk.code[sp * stk.stride] = c;
+      stk.tval[sp * stk.stride] = entryT<R>(t);
+      ++sp;
+    };
+    for (;;) {
+      if (sp == 0) {
+        if (nsp == 0) break;
+        nsp -= kHalf;
+        for (int i = 0; i < kHalf; ++i) {
+          stk.code[i * stk.stride] = spill_code[nsp + i];
+          stk.tval[i * stk.stride] = spill_t[nsp + i];
+        }
+        sp = kHalf;
+      }
+      --sp;
+      const int code = stk.code[sp * stk.stride];
+      const float et = stk.tval[sp * stk.stride];
+      if (R(et) > closest) continue;
+      if (code >= 0) {
+        const DNode n = sc.nodes[code];
+        R tl, tr;
+        const bool hl = slab<R>(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, o, inv, nx, ny, nz, tmin, closest, &tl) &&
+                        n.d.x != kNoChild;
+        const bool hr = slab<R>(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, o, inv, nx, ny, nz, tmin, closest, &tr) &&
+                        n.d.y != kNoChild;
+        if (hl && hr) {
+          const bool near_left = tl <= tr;
+          push(near_left ? n.d.y : n.d.x, near_left ? tr : tl);
+          push(near_left ? n.d.x : n.d.y, near_left ? tl : tr);
+        } else if (hl) {
+          push(n.d.x, tl);
+        } else if (hr) {
+          push(n.d.y, tr);
+        }
+      } else {
+        const int v = -1 - code;
+        const int first = v >> 4, count = v & 15;
+        for (int k = first; k < first + count; ++k) {
+          if (kIdentity && k == excl) continue;
+          const Prim<R> p = loadPrim<R>(sc, k);
+          R t;
+          bool h;
+          if (metaKind(p.meta) == kKindTriangle) {
+            h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
+          } else {
+            h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
+                                         : sphereHit<R>(p, o, d, tmin, closest, &t);
+          }
+          if (h) {
+            closest = t;
+            best = k;
+            if (anyhit) {
+              done = true;
+              break;
+            }
+          }
+        }
+        if (done) break;
+      }
+    }
+  }
+  if (!done) {
+    for (int j = 0; j < sc.num_planes; ++j) {
+      const int id = sc.num_prims + j;
+      if (kIdentity && (id == self || id == excl)) continue;
+      R t;
+      if (planeHit<R>(sc.planes[j], o, d, tmin, closest, &t)) {
+        closest = t;
+        best = id;
+        if (anyhit) break;
+      }
+    }
+  }
+  return HitRec<R>{closest, best};
+}
+
+// Brute force over every primitive in slot order, then planes
+// (geometry.zig:129-139 and scene.zig:109-117): the reference the BVH
+// traversal is checked against on the device, and the path for scenes
+// handed over without a BVH.
+template <typename R>
+ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tmax, int self, int excl,
+                          bool anyhit) {
+  constexpr bool kIdentity = !K<R>::exact;
+  R closest = tmax;
+  int best = -1;
+  for (int k = 0; k < sc.num_prims; ++k) {
+    if (kIdentity && k == excl) continue;
+    const Prim<R> p = loadPrim<R>(sc, k);
+    R t;
+    bool h;
+    if (metaKind(p.meta) == kKindTriangle) {
+      h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
+    } else {
+      h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
+                                   : sphereHit<R>(p, o, d, tmin, closest, &t);
+    }
+    if (h) {
+      closest = t;
+      best = k;
+      if (anyhit) return HitRec<R>{closest, best};
+    }
+  }
+  for (int j = 0; j < sc.num_planes; ++j) {
+    const int id = sc.num_prims + j;
+    if (kIdentity && (id == self || id == excl)) continue;
+    R t;
+    if (planeHit<R>(sc.planes[j], o, d, tmin, closest, &t)) {
+      closest = t;
+      best = id;
+      if (anyhit) break;
+    }
+  }
+  return HitRec<R>{closest, best};
+}
+
+// Surface data of a hit (geometry.zig:20-29): point, ray-facing normal,
+// front-face flag and material.
+template <typename R>
+struct Surface {
+  V3<R> p, n;
+  bool front;
+  int material;
+};
+
+template <typename R>
+ZT_D Surface<R> surfaceAt(const SceneView<R>& sc, V3<R> o, V3<R> d, HitRec<R> h) {
+  Surface<R> s;
+  s.p = add(o, scale(d, h.t));
+  V3<R> outward;
+  if (h.prim >= sc.num_prims) {
+    const Plane<R>& pl = sc.planes[h.prim - sc.num_prims];
+    outward = pl.normal;
+    s.material = pl.material;
+  } else {
+    const Prim<R> p = loadPrim<R>(sc, h.prim);
+    s.material = (int)metaMaterial(p.meta);
+    if (metaKind(p.meta) == kKindTriangle) {
+      outward = normalize(cross(p.b, p.c));
+    } else {
+      outward = vdiv(sub(s.p, p.a), p.b.x);
+    }
+  }
+  s.front = dot(d, outward) < 0;
+  s.n = s.front ? outward : neg(outward);
+  return s;
+}
+
+}  // namespace ztd

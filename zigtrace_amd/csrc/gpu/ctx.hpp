@@ -1,0 +1,131 @@
+// Internal state behind the zt_gpu.h handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../device/dev_common.hpp"
+#include "zt_gpu.h"
+
+namespace ztg {
+
+// Thrown inside the library, converted to a status code at the ABI edge.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define ZT_HIP(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      throw ::ztg::Error(ZT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+// A device allocation that frees itself.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    reset();
+    if (n == 0) return;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(e == hipErrorOutOfMemory ? ZT_ERR_OUT_OF_MEMORY : ZT_ERR_HIP,
+                  std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    bytes = n;
+  }
+  // Grow-only scratch.
+  void ensure(size_t n) {
+    if (n > bytes) alloc(n);
+  }
+  template <typename T>
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) ZT_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// One uploaded scene: the device layout in both precisions plus what the
+// host needs to rebuild per-render constants.
+struct DevScene {
+  DevBuf nodes, primsF, primsD, planesF, planesD, matsF, matsD, lightsF, lightsD, cdf, prim_light;
+  int num_nodes = 0, num_prims = 0, num_planes = 0, num_lights = 0, num_mats = 0;
+  int tree_depth = 0;
+  bool brute_force = false;  // no BVH handed over (or one too deep for the stack)
+  double light_total = 0;
+  zt_camera_config camera{};
+  zt_background background{};
+  // Host copies used to rebuild material/light tables when parameters change.
+  std::vector<zt_material> materials;
+  std::vector<ztd::Light<double>> lights_host;
+  std::vector<double> light_area_lum;  // per light: area (weight = luminance(Le) * area)
+  std::vector<int> light_material;     // per light: material id of the emitter
+};
+
+}  // namespace ztg
+
+struct zt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::vector<std::unique_ptr<ztg::DevScene>> scenes;  // index = scene id; null = released
+  zt_gpu_stats stats{};
+  ztg::DevBuf out, errflag, scratch;
+};
+
+namespace ztg {
+
+DevScene& sceneOf(zt_ctx* ctx, uint32_t id);
+void setDevice(zt_ctx* ctx);
+
+// Camera.init (camera.zig:29-55) in f64 on the host.
+ztd::CamD cameraInit(const zt_camera_config& cfg, double aspect);
+
+template <typename R>
+ztd::SceneView<R> viewOf(const DevScene& s);
+
+// Runs `f`, converting exceptions to a status code and recording the message.
+template <typename F>
+int guarded(zt_ctx* ctx, F&& f) {
+  try {
+    f();
+    if (ctx) ctx->err.clear();
+    return ZT_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return ZT_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return ZT_ERR_INTERNAL;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown exception";
+    return ZT_ERR_INTERNAL;
+  }
+}
+
+}  // namespace ztg

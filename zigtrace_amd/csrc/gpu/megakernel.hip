@@ -1,0 +1,355 @@
+// The plain renderer: one lane per pixel (replacing the tile loop of
+// render.zig:104-180), plus the ray-level entry points used for parity
+// (traceWith, Scene.intersect, the RNG streams).
+//
+// A lane owns its pixel's xoshiro256++ stream (render.zig:135) and its
+// accumulator, so images need no atomics and are bit-identical for any launch
+// shape. The lane runs a flat loop of ray queries: each iteration traces one
+// extension or shadow ray, then shades or resolves it; when a path ends the
+// lane starts its pixel's next sample in the same iteration structure, so a
+// wave keeps tracing while its lanes' paths have different lengths.
+
+#include <cmath>
+
+#include "../device/dev_path.hpp"
+#include "ctx.hpp"
+#include "launch.hpp"
+
+using namespace ztd;
+
+namespace ztg {
+
+static constexpr double kRayEpsilon = 1e-4;  // render.zig:26
+
+template <typename R>
+ZT_D HitRec<R> query(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V3<R> d, R tmax, int self, int excl,
+                     bool anyhit, int* overflow) {
+  if (sc.num_nodes == 0 && sc.num_prims > 0) return bruteForce<R>(sc, o, d, R(kRayEpsilon), tmax, self, excl, anyhit);
+  return traverse<R>(sc, stk, o, d, R(kRayEpsilon), tmax, self, excl, anyhit, overflow);
+}
+
+// Maps block b to a pixel tile so that the blocks the hardware places on one
+// XCD (b mod 8) cover one contiguous band of tiles, sharing that XCD's L2.
+ZT_D uint32_t xcdTile(uint32_t b, uint32_t nblocks) {
+  const uint32_t q = nblocks / 8, r = nblocks % 8;
+  const uint32_t x = b % 8, i = b / 8;
+  return x * q + min(x, r) + i;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void megakernel(SceneView<R> sc, Params pr, R* __restrict__ out, uint32_t tile_w,
+                                                  uint32_t tile_h, uint32_t tiles_x, uint32_t ntiles, int* err) {
+  extern __shared__ int lds[];
+  const uint32_t tid = threadIdx.x;
+  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const uint32_t tile = xcdTile(blockIdx.x, ntiles);
+  const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
+  const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
+  if (x >= pr.width || y >= pr.height) return;
+  const uint64_t pix = (uint64_t)y * pr.width + x;
+  Rng rng = rngForStream(pr.seed, pix);
+  V3<R> sum = splat(R(0));
+  Path<R> st;
+  uint32_t sample = 0;
+  int phase = kCamera;
+  int overflow = 0;
+  for (;;) {
+    if (phase == kCamera) {
+      if (sample == pr.spp) break;
+      phase = beginPath<R>(sc, pr, x, y, rng, st);
+    }
+    if (phase == kExtend || phase == kShadow) {
+      const bool sh = phase == kShadow;
+      const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
+                                   sh ? st.sh_excl : -1, sh, &overflow);
+      phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
+    }
+    if (phase == kFinish) {
+      if (isFinite(st.radiance)) sum = add(sum, st.radiance);  // render.zig:141-142
+      ++sample;
+      phase = kCamera;
+    }
+  }
+  const R inv_spp = R(1) / R(pr.spp);
+  out[3 * pix + 0] = sum.x * inv_spp;
+  out[3 * pix + 1] = sum.y * inv_spp;
+  out[3 * pix + 2] = sum.z * inv_spp;
+  if (overflow) atomicOr(err, 1);
+}
+
+// traceWith for caller-given rays and RNG states (render.zig:61-102).
+template <typename R>
+__global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, const zt_ray* rays, uint64_t n,
+                                                   uint64_t* states, double* out, int* err) {
+  extern __shared__ int lds[];
+  const uint32_t tid = threadIdx.x;
+  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
+  if (i >= n) return;
+  Rng rng{states[4 * i], states[4 * i + 1], states[4 * i + 2], states[4 * i + 3]};
+  Path<R> st;
+  st.ro = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
+  st.rd = mk((R)rays[i].dir[0], (R)rays[i].dir[1], (R)rays[i].dir[2]);
+  st.radiance = splat(R(0));
+  st.throughput = splat(R(1));
+  st.prev_origin = splat(R(0));
+  st.prev_pdf = R(0);
+  st.self = -1;
+  st.sh_excl = -1;
+  st.depth = 0;
+  st.terminated = 0;
+  int phase = pr.max_depth == 0 ? kFinish : kExtend;
+  int overflow = 0;
+  while (phase != kFinish) {
+    const bool sh = phase == kShadow;
+    const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
+                                 sh ? st.sh_excl : -1, sh, &overflow);
+    phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
+  }
+  out[3 * i + 0] = st.radiance.x;
+  out[3 * i + 1] = st.radiance.y;
+  out[3 * i + 2] = st.radiance.z;
+  states[4 * i + 0] = rng.s0;
+  states[4 * i + 1] = rng.s1;
+  states[4 * i + 2] = rng.s2;
+  states[4 * i + 3] = rng.s3;
+  if (overflow) atomicOr(err, 1);
+}
+
+// Scene.intersect / brute force for caller-given rays (no self exclusion:
+// these rays do not start on a surface).
+template <typename R>
+__global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt_ray* rays, uint64_t n, R tmin, R tmax,
+                                                       int mode, zt_hit* out, int* err) {
+  extern __shared__ int lds[];
+  const uint32_t tid = threadIdx.x;
+  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
+  if (i >= n) return;
+  const V3<R> o = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
+  const V3<R> d = mk((R)rays[i].dir[0], (R)rays[i].dir[1], (R)rays[i].dir[2]);
+  int overflow = 0;
+  const HitRec<R> h = (mode == 1 || (sc.num_nodes == 0 && sc.num_prims > 0))
+                          ? bruteForce<R>(sc, o, d, tmin, tmax, -1, -1, false)
+                          : traverse<R>(sc, stk, o, d, tmin, tmax, -1, -1, false, &overflow);
+  zt_hit r{};
+  r.hit = h.prim >= 0 ? 1 : 0;
+  if (h.prim >= 0) {
+    const Surface<R> s = surfaceAt<R>(sc, o, d, h);
+    r.t = h.t;
+    r.point[0] = s.p.x;
+    r.point[1] = s.p.y;
+    r.point[2] = s.p.z;
+    r.normal[0] = s.n.x;
+    r.normal[1] = s.n.y;
+    r.normal[2] = s.n.z;
+    r.front_face = s.front ? 1 : 0;
+    r.material = (uint32_t)s.material;
+    r._pad = h.prim;
+  }
+  out[i] = r;
+  if (overflow) atomicOr(err, 1);
+}
+
+__global__ void rngKernel(uint64_t seed, const uint64_t* streams, uint64_t n, uint32_t draws, int kind,
+                          uint64_t* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Rng r = rngForStream(seed, streams[i]);
+  for (uint32_t k = 0; k < draws; ++k) {
+    uint64_t v;
+    if (kind == 0) v = r.next();
+    else v = (uint64_t)__float_as_uint(r.u32f());
+    out[i * draws + k] = v;
+  }
+}
+
+static size_t ldsBytes(uint32_t threads) { return (size_t)2 * kLds * threads * sizeof(int); }
+
+Params makeParams(const DevScene& s, const zt_render_options& o) {
+  Params p{};
+  p.seed = o.seed;
+  p.width = (uint32_t)o.width;
+  p.height = (uint32_t)o.height;
+  p.spp = (uint32_t)o.spp;
+  p.max_depth = (uint32_t)std::min<uint64_t>(o.max_depth, 0xFFFFFFFFull);
+  p.use_nee = (o.light_sampling != 0 && s.num_lights > 0) ? 1 : 0;
+  return p;
+}
+
+void checkOptions(const zt_render_options* o) {
+  if (!o) throw Error(ZT_ERR_INVALID_ARGUMENT, "null options");
+  // render.zig:153, checked before anything else.
+  if (o->width == 0 || o->height == 0 || o->spp == 0 || o->tile_size == 0)
+    throw Error(ZT_ERR_INVALID_OPTIONS, "width, height, spp and tile_size must be non-zero");
+  if (o->width > (1u << 20) || o->height > (1u << 20) || o->width * o->height > (1ull << 31) ||
+      o->spp > 0xFFFFFFFFull)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "image size or spp beyond the device limits");
+}
+
+template <typename R>
+void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout) {
+  const SceneView<R> view = viewOf<R>(s);
+  Params pr = makeParams(s, o);
+  uint32_t tw = tu.tile_w ? tu.tile_w : 8;
+  uint32_t th = tu.tile_h ? tu.tile_h : 32;
+  const uint32_t threads = tw * th;
+  if (threads == 0 || threads > 256 || threads % 64 != 0)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel tile must hold 64, 128, 192 or 256 pixels");
+  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
+  const uint32_t tiles_y = (pr.height + th - 1) / th;
+  const uint32_t ntiles = tiles_x * tiles_y;
+  hipLaunchKernelGGL(megakernel<R>, dim3(ntiles), dim3(threads), ldsBytes(threads), ctx->stream, view, pr, dout, tw,
+                     th, tiles_x, ntiles, ctx->errflag.as<int>());
+  ZT_HIP(hipGetLastError());
+  ctx->stats.launches += 1;
+}
+template void launchMegakernel<float>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
+                                      float*);
+template void launchMegakernel<double>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
+                                       double*);
+
+void resetErr(zt_ctx* ctx) { ZT_HIP(hipMemsetAsync(ctx->errflag.p, 0, sizeof(int) * 4, ctx->stream)); }
+
+void checkErr(zt_ctx* ctx) {
+  int e = 0;
+  ZT_HIP(hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ZT_HIP(hipStreamSynchronize(ctx->stream));
+  if (e) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+}
+
+}  // namespace ztg
+
+using namespace ztg;
+
+extern "C" {
+
+int zt_gpu_render(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
+                  void* out, int out_on_device) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    checkOptions(opts);
+    if (!out) throw Error(ZT_ERR_INVALID_ARGUMENT, "null output");
+    DevScene& s = sceneOf(ctx, scene_id);
+    zt_gpu_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.precision > ZT_PRECISION_FP64 || tu.method > ZT_METHOD_WAVEFRONT)
+      throw Error(ZT_ERR_INVALID_ARGUMENT, "bad tuning enum");
+    setDevice(ctx);
+    const bool f64 = tu.precision == ZT_PRECISION_FP64;
+    const size_t elem = f64 ? sizeof(double) : sizeof(float);
+    const size_t bytes = (size_t)opts->width * opts->height * 3 * elem;
+    void* dout = out;
+    if (!out_on_device) {
+      ctx->out.ensure(bytes);
+      dout = ctx->out.p;
+    }
+    ctx->stats.launches = 0;
+    resetErr(ctx);
+    ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const uint32_t method = tu.method == ZT_METHOD_AUTO ? ZT_METHOD_MEGAKERNEL : tu.method;
+    if (method == ZT_METHOD_WAVEFRONT) {
+      if (f64) launchWavefront<double>(ctx, s, *opts, tu, (double*)dout);
+      else launchWavefront<float>(ctx, s, *opts, tu, (float*)dout);
+    } else {
+      if (f64) launchMegakernel<double>(ctx, s, *opts, tu, (double*)dout);
+      else launchMegakernel<float>(ctx, s, *opts, tu, (float*)dout);
+    }
+    ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!out_on_device) ZT_HIP(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    checkErr(ctx);
+    float ms = 0;
+    ZT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.kernel_ms = ms;
+    ctx->stats.paths = (uint64_t)opts->width * opts->height * opts->spp;
+  });
+}
+
+int zt_gpu_trace_rays(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, uint64_t* rng_states,
+                      uint64_t max_depth, uint32_t light_sampling, uint32_t precision, double* out_rgb) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (n == 0) return;
+    if (!rays || !rng_states || !out_rgb) throw Error(ZT_ERR_INVALID_ARGUMENT, "null buffer");
+    if (precision > ZT_PRECISION_FP64) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad precision");
+    DevScene& s = sceneOf(ctx, scene_id);
+    setDevice(ctx);
+    zt_render_options o{};
+    o.max_depth = max_depth;
+    o.light_sampling = light_sampling;
+    const Params pr = makeParams(s, o);
+    DevBuf drays, dstates, dout;
+    drays.alloc(n * sizeof(zt_ray));
+    dstates.alloc(n * 32);
+    dout.alloc(n * 3 * sizeof(double));
+    ZT_HIP(hipMemcpyAsync(drays.p, rays, n * sizeof(zt_ray), hipMemcpyHostToDevice, ctx->stream));
+    ZT_HIP(hipMemcpyAsync(dstates.p, rng_states, n * 32, hipMemcpyHostToDevice, ctx->stream));
+    resetErr(ctx);
+    const uint32_t threads = 256;
+    const uint32_t blocks = (uint32_t)((n + threads - 1) / threads);
+    if (precision == ZT_PRECISION_FP64)
+      hipLaunchKernelGGL(traceKernel<double>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
+                         viewOf<double>(s), pr, drays.as<zt_ray>(), n, dstates.as<uint64_t>(), dout.as<double>(),
+                         ctx->errflag.as<int>());
+    else
+      hipLaunchKernelGGL(traceKernel<float>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
+                         viewOf<float>(s), pr, drays.as<zt_ray>(), n, dstates.as<uint64_t>(), dout.as<double>(),
+                         ctx->errflag.as<int>());
+    ZT_HIP(hipGetLastError());
+    ZT_HIP(hipMemcpyAsync(out_rgb, dout.p, n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZT_HIP(hipMemcpyAsync(rng_states, dstates.p, n * 32, hipMemcpyDeviceToHost, ctx->stream));
+    checkErr(ctx);
+  });
+}
+
+int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, double t_min, double t_max,
+                     int mode, uint32_t precision, zt_hit* out) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (n == 0) return;
+    if (!rays || !out) throw Error(ZT_ERR_INVALID_ARGUMENT, "null buffer");
+    if (precision > ZT_PRECISION_FP64 || mode < 0 || mode > 1) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad enum");
+    DevScene& s = sceneOf(ctx, scene_id);
+    setDevice(ctx);
+    DevBuf drays, dout;
+    drays.alloc(n * sizeof(zt_ray));
+    dout.alloc(n * sizeof(zt_hit));
+    ZT_HIP(hipMemcpyAsync(drays.p, rays, n * sizeof(zt_ray), hipMemcpyHostToDevice, ctx->stream));
+    resetErr(ctx);
+    const uint32_t threads = 256;
+    const uint32_t blocks = (uint32_t)((n + threads - 1) / threads);
+    if (precision == ZT_PRECISION_FP64)
+      hipLaunchKernelGGL(intersectKernel<double>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
+                         viewOf<double>(s), drays.as<zt_ray>(), n, t_min, t_max, mode, dout.as<zt_hit>(),
+                         ctx->errflag.as<int>());
+    else
+      hipLaunchKernelGGL(intersectKernel<float>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
+                         viewOf<float>(s), drays.as<zt_ray>(), n, (float)t_min, (float)t_max, mode,
+                         dout.as<zt_hit>(), ctx->errflag.as<int>());
+    ZT_HIP(hipGetLastError());
+    ZT_HIP(hipMemcpyAsync(out, dout.p, n * sizeof(zt_hit), hipMemcpyDeviceToHost, ctx->stream));
+    checkErr(ctx);
+  });
+}
+
+int zt_gpu_rng(zt_ctx* ctx, uint64_t seed, const uint64_t* streams, uint64_t n, uint32_t draws, int kind,
+               uint64_t* out) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (n == 0 || draws == 0) return;
+    if (!streams || !out || kind < 0 || kind > 1) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad argument");
+    setDevice(ctx);
+    DevBuf ds, dout;
+    ds.alloc(n * 8);
+    dout.alloc(n * draws * 8);
+    ZT_HIP(hipMemcpyAsync(ds.p, streams, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t threads = 256;
+    hipLaunchKernelGGL(rngKernel, dim3((uint32_t)((n + threads - 1) / threads)), dim3(threads), 0, ctx->stream, seed,
+                       ds.as<uint64_t>(), n, draws, kind, dout.as<uint64_t>());
+    ZT_HIP(hipGetLastError());
+    ZT_HIP(hipMemcpyAsync(out, dout.p, n * draws * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"
