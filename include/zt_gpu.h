@@ -88,6 +88,12 @@ const char* zt_gpu_last_error(const zt_ctx* ctx);
 void* zt_gpu_stream(zt_ctx* ctx);
 int zt_gpu_get_stats(const zt_ctx* ctx, zt_gpu_stats* out);
 
+/* Device memory owned by the caller (e.g. an output framebuffer that stays
+ * in HBM). No reference counterpart: the reference returns a host Image. */
+int zt_gpu_device_alloc(zt_ctx* ctx, uint64_t bytes, void** out);
+int zt_gpu_device_free(zt_ctx* ctx, void* ptr);
+int zt_gpu_copy_to_host(zt_ctx* ctx, void* host, const void* device, uint64_t bytes);
+
 /* Scenes ----------------------------------------------------------------- */
 /* Copies a built scene to the device in the GPU layout (64-byte two-child
  * BVH nodes, leaf-ordered primitives, fp32 and on request fp64 copies).

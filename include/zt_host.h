@@ -41,6 +41,8 @@ void zt_scene_destroy(zt_scene* scene);
 int zt_scene_add_material(zt_scene* scene, const zt_material* m, uint32_t* id_out);
 /* Scene.add (scene.zig:71-78): planes go to the unbounded list. Invalidates the build. */
 int zt_scene_add(zt_scene* scene, const zt_shape* shape);
+/* Scene.add for n shapes in order (one call instead of n; same result). */
+int zt_scene_add_many(zt_scene* scene, const zt_shape* shapes, uint64_t n);
 /* Scene.addQuad (scene.zig:82-88): corner + a*u + b*v as two triangles, front face on u x v. */
 int zt_scene_add_quad(zt_scene* scene, const double corner[3], const double u[3], const double v[3],
                       uint32_t material);

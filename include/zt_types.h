@@ -111,7 +111,7 @@ typedef struct zt_hit {
   uint32_t front_face;
   uint32_t material;
   int32_t hit;
-  int32_t _pad;
+  int32_t prim; /* GPU results: device primitive slot hit (planes after the BVH primitives) */
 } zt_hit;
 
 #ifdef __cplusplus

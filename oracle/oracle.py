@@ -294,6 +294,18 @@ class OracleScene:
         return s
 
     @classmethod
+    def from_view_arrays(cls, a: dict, build: bool = True) -> "OracleScene":
+        """From the flattened scene inputs (materials, bounded shapes, planes,
+        camera, background) of the product's Scene.view_arrays(): the oracle
+        runs its own Bvh.build and Lights.build on them."""
+        shapes = np.concatenate([a["shapes"], a["planes"]]).view(SHAPE)
+        c = a["camera"][0]
+        cam = _rec(CAMERA, **{"from": c["look_from"], "at": c["look_at"], "up": c["up"], "vfov": c["vfov"],
+                              "aperture": c["aperture"], "focus": c["focus_dist"], "has_focus": c["has_focus_dist"]})
+        bg = np.ascontiguousarray(a["background"]).view(BACKGROUND).copy()
+        return cls.from_arrays(np.ascontiguousarray(a["materials"]).view(MATERIAL), shapes, cam, bg, build=build)
+
+    @classmethod
     def parse(cls, text: str | bytes) -> "OracleScene":
         b = text.encode() if isinstance(text, str) else text
         out = C.c_void_p()

@@ -64,7 +64,7 @@ template <typename R> ZT_HD V3<R> refract(V3<R> uv, V3<R> n, R eta) {  // vec.zi
   return add(r_perp, r_par);
 }
 template <typename R> ZT_D R getc(V3<R> a, int axis) { return axis == 0 ? a.x : (axis == 1 ? a.y : a.z); }
-template <typename T, typename R> ZT_D V3<T> cvt(V3<R> a) { return V3<T>{(T)a.x, (T)a.y, (T)a.z}; }
+template <typename T, typename R> ZT_HD V3<T> cvt(V3<R> a) { return V3<T>{(T)a.x, (T)a.y, (T)a.z}; }
 
 template <typename R> struct K;
 template <> struct K<double> {

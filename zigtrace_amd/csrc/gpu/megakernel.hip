@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
     r.normal[2] = s.n.z;
     r.front_face = s.front ? 1 : 0;
     r.material = (uint32_t)s.material;
-    r._pad = h.prim;
+    r.prim = h.prim;
   }
   out[i] = r;
   if (overflow) atomicOr(err, 1);

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "../device/dev_trace.hpp"
 #include "ctx.hpp"
 
 using namespace ztd;
@@ -322,6 +323,37 @@ int zt_gpu_get_stats(const zt_ctx* ctx, zt_gpu_stats* out) {
   if (!ctx || !out) return ZT_ERR_INVALID_ARGUMENT;
   *out = ctx->stats;
   return ZT_OK;
+}
+
+int zt_gpu_device_alloc(zt_ctx* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    setDevice(ctx);
+    *out = nullptr;
+    const hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+      *out = nullptr;
+      throw Error(e == hipErrorOutOfMemory ? ZT_ERR_OUT_OF_MEMORY : ZT_ERR_HIP, hipGetErrorString(e));
+    }
+  });
+}
+
+int zt_gpu_device_free(zt_ctx* ctx, void* ptr) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    setDevice(ctx);
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+    if (ptr) ZT_HIP(hipFree(ptr));
+  });
+}
+
+int zt_gpu_copy_to_host(zt_ctx* ctx, void* host, const void* device, uint64_t bytes) {
+  if (!ctx || (bytes && (!host || !device))) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    setDevice(ctx);
+    ZT_HIP(hipMemcpyAsync(host, device, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+  });
 }
 
 int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id) {

@@ -60,12 +60,12 @@ struct DeviceCopy {
   uint64_t version;
   uint32_t id;
 };
+// The context lives until the process ends: destroying it from a
+// thread_local destructor at exit could run after the HIP runtime's own
+// teardown.
 struct ThreadGpu {
   zt_ctx* ctx = nullptr;
   std::vector<DeviceCopy> copies;
-  ~ThreadGpu() {
-    if (ctx) zt_gpu_destroy(ctx);
-  }
 };
 thread_local ThreadGpu g_gpu;
 
@@ -102,6 +102,18 @@ int zt_scene_add(zt_scene* s, const zt_shape* sh) {
   s->invalidate();
   if (sh->kind == ZT_PLANE) s->planes.push_back(*sh);
   else s->shapes.push_back(*sh);
+  return ZT_OK;
+}
+
+int zt_scene_add_many(zt_scene* s, const zt_shape* sh, uint64_t n) {
+  if (!s || (n && !sh)) return ZT_ERR_INVALID_ARGUMENT;
+  for (uint64_t i = 0; i < n; ++i)
+    if (sh[i].kind > ZT_PLANE) return ZT_ERR_INVALID_ARGUMENT;
+  s->invalidate();
+  for (uint64_t i = 0; i < n; ++i) {
+    if (sh[i].kind == ZT_PLANE) s->planes.push_back(sh[i]);
+    else s->shapes.push_back(sh[i]);
+  }
   return ZT_OK;
 }
 
