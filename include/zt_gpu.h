@@ -110,6 +110,14 @@ int zt_gpu_scene_release(zt_ctx* ctx, uint32_t scene_id);
 int zt_gpu_render(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
                   void* out, int out_on_device);
 
+/* Many renders in one persistent wavefront pass (no reference counterpart:
+ * the reference renders one scene per call). Scene i is rendered with
+ * opts[i] into outs[i] exactly as zt_gpu_render would (same bits); a shared
+ * pool of path slots moves across scenes as pixels complete, so scenes of
+ * uneven size, spp and depth share the GPU without per-scene launches. */
+int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, const zt_render_options* opts,
+                        const zt_gpu_tuning* tuning, void* const* outs, int out_on_device);
+
 /* traceWith (render.zig:61-102) for n caller-given rays, each with its own
  * xoshiro256++ state (4 x u64 per ray, advanced in place as the reference
  * advances `*Rng`). Radiance goes to out_rgb (n*3 doubles). Host buffers. */
@@ -127,6 +135,28 @@ int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_
  * pattern in the low 32 bits. */
 int zt_gpu_rng(zt_ctx* ctx, uint64_t seed, const uint64_t* streams, uint64_t n, uint32_t draws, int kind,
                uint64_t* out);
+
+/* Differentiable rendering ---------------------------------------------- */
+/* No reference counterpart (the reference is not differentiable): the
+ * forward/backward pair of an autograd Function over the scene's colours.
+ * Parameters, 3 floats each in material order: the albedo of every
+ * Lambertian and metal material and the radiance of every emissive material
+ * (dielectrics have none). Setting them updates the materials and rebuilds
+ * the light table as Lights.build would (light.zig:68-94). */
+int zt_gpu_param_count(zt_ctx* ctx, uint32_t scene_id, uint32_t* count);
+int zt_gpu_param_offsets(zt_ctx* ctx, uint32_t scene_id, int32_t* offsets); /* per material, -1: none */
+int zt_gpu_get_params(zt_ctx* ctx, uint32_t scene_id, float* params);
+int zt_gpu_set_params(zt_ctx* ctx, uint32_t scene_id, const float* params);
+/* Forward: set `params` (if not NULL) and render (as zt_gpu_render). */
+int zt_gpu_forward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
+                   const float* params, void* out, int out_on_device);
+/* Backward: dLoss/dparams for dLoss/dimage (width*height*3 floats, the
+ * adjoint of the forward image) at `params` (if not NULL) and the same
+ * options and seed. Re-traces every path from its pixel's RNG stream; keeps
+ * nothing from the forward call. Requires max_depth <= 32. `grad_on_device`
+ * != 0: dL_dimage is device memory. Result: num_params doubles (host). */
+int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
+                    const float* params, const float* dL_dimage, int grad_on_device, double* dL_dparams);
 
 #ifdef __cplusplus
 }

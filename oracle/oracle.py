@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
         _lib.orc_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p,
                                    C.c_void_p]
         _lib.orc_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        _lib.orc_render_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
         _lib.orc_rng_state.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
         _lib.orc_rng_draw.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_void_p]
         _lib.orc_splitmix64.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
@@ -397,6 +399,25 @@ class OracleScene:
         if rc:
             raise ValueError("InvalidOptions")
         return out, ctr[0]
+
+
+def render_grad(scene: "OracleScene", width, height, spp, max_depth, offsets, dimg, seed=0, threads=None,
+                light_sampling=True):
+    """FP64 reference gradient (forward-mode dual numbers): (image, dLoss/dparams)."""
+    if threads is None:
+        threads = hardware_threads()
+    o = _rec(OPTIONS, width=width, height=height, spp=spp, max_depth=max_depth, threads=threads, seed=seed,
+             tile_size=16, light_sampling=int(light_sampling))
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    nparams = int(offsets.max()) + 3 if (offsets >= 0).any() else 0
+    dimg = np.ascontiguousarray(dimg, dtype=np.float64).reshape(-1)
+    assert dimg.size == width * height * 3
+    img = np.zeros((height, width, 3))
+    grad = np.zeros(max(nparams, 1))
+    rc = lib().orc_render_grad(scene.h, _p(o), _p(offsets), nparams, _p(dimg), _p(img), _p(grad))
+    if rc:
+        raise ValueError("InvalidOptions" if rc == 1 else "too many parameters")
+    return img, grad[:nparams]
 
 
 def hardware_threads() -> int:

@@ -618,6 +618,7 @@ struct LightSample {
   double distance;
   V3 radiance;
   double pdf;
+  size_t index = 0;  // which light was picked (not part of the reference's struct)
 };
 struct Lights {
   std::vector<Light> items;
@@ -659,6 +660,7 @@ struct Lights {
   bool sample(V3 origin, Rng* rng, LightSample* out) const {  // light.zig:120-166
     if (items.empty()) return false;
     const Light& light = pick(rng);
+    out->index = (size_t)(&light - items.data());
     const double select = light.weight / total;
     const Shape& s = light.shape;
     if (s.kind == kTriangle) {
@@ -672,7 +674,7 @@ struct Lights {
       const V3 n = normalize(cross(sub(s.b, s.a), sub(s.c, s.a)));
       const double cos_l = -dot(dir, n);
       if (cos_l <= 1e-9) return false;
-      *out = {dir, dist, light.radiance, select * dist * dist / (cos_l * triangleArea(s))};
+      *out = {dir, dist, light.radiance, select * dist * dist / (cos_l * triangleArea(s)), out->index};
       return true;
     }
     // sphere
@@ -691,7 +693,7 @@ struct Lights {
     Hit h;
     if (!sphereIntersect(s, Ray{origin, dir}, 0, kInf, &h)) return false;
     if (!h.front) return false;
-    *out = {dir, h.t, light.radiance, select / (2 * M_PI * (1 - cos_max))};
+    *out = {dir, h.t, light.radiance, select / (2 * M_PI * (1 - cos_max)), out->index};
     return true;
   }
   double pdfHit(V3 origin, const Hit& hit) const {  // light.zig:171-198
@@ -836,6 +838,115 @@ static V3 traceWith(const Scene& scene, Ray r, Rng* rng, uint64_t max_depth, boo
     }
   }
   return radiance;
+}
+
+// ---------------------------------------------------------------- gradient (dual numbers)
+// Forward-mode derivative of traceWith (render.zig:61-102) with respect to
+// the material colours: the albedo of Lambertian/metal materials and the
+// radiance of emissive ones (3 parameters each, offsets per material). The
+// value part repeats traceWith operation for operation. Geometry, visibility,
+// MIS weights and light-selection probabilities are constants; the Russian
+// roulette probability is used by value only (the estimator T/p*[u<p] has
+// expectation T for any p, so its derivative is taken with p fixed).
+static const int kMaxGradParams = 96;
+struct DV3 {
+  V3 v;
+  V3 d[kMaxGradParams];
+};
+struct GradCtx {
+  const int32_t* offsets;  // per material: first parameter index or -1
+  int nparams;
+};
+static inline void dvZero(DV3& a, int np, V3 v) {
+  a.v = v;
+  for (int k = 0; k < np; ++k) a.d[k] = splat(0);
+}
+// a += b (colour contribution with derivatives)
+static inline void dvAddInto(DV3& a, const DV3& b, int np) {
+  a.v = add(a.v, b.v);
+  for (int k = 0; k < np; ++k) a.d[k] = add(a.d[k], b.d[k]);
+}
+// unit derivative of material m's colour channel c
+static inline V3 unitChan(int c, double x) { return c == 0 ? v3(x, 0, 0) : (c == 1 ? v3(0, x, 0) : v3(0, 0, x)); }
+
+static V3 traceWithGrad(const Scene& scene, Ray r, Rng* rng, uint64_t max_depth, bool light_sampling,
+                        const GradCtx& g, DV3* out) {
+  const int np = g.nparams;
+  const bool use_nee = light_sampling && !scene.lights.items.empty();
+  DV3 radiance, throughput, tmp;
+  dvZero(radiance, np, splat(0));
+  dvZero(throughput, np, splat(1));
+  V3 prev_origin = splat(0);
+  double prev_pdf = 0;
+  for (uint64_t depth = 0; depth < max_depth; ++depth) {
+    Hit hit;
+    if (!scene.intersect(r, kRayEpsilon, kInf, &hit, nullptr)) {
+      const V3 bg = scene.background.radiance(r.d);
+      radiance.v = add(radiance.v, mul(throughput.v, bg));
+      for (int k = 0; k < np; ++k) radiance.d[k] = add(radiance.d[k], mul(throughput.d[k], bg));
+      *out = radiance;
+      return radiance.v;
+    }
+    const Material& mat = scene.materials[hit.material];
+    const V3 le = emitted(mat, hit);
+    if (maxComponent(le) > 0) {
+      const double w = prev_pdf > 0 ? powerWeight(prev_pdf, scene.lights.pdfHit(prev_origin, hit)) : 1.0;
+      radiance.v = add(radiance.v, scale(mul(throughput.v, le), w));
+      for (int k = 0; k < np; ++k) radiance.d[k] = add(radiance.d[k], scale(mul(throughput.d[k], le), w));
+      const int off = g.offsets[hit.material];
+      if (off >= 0)
+        for (int c = 0; c < 3; ++c)
+          radiance.d[off + c] = add(radiance.d[off + c], scale(mul(throughput.v, unitChan(c, 1.0)), w));
+    }
+    if (mat.kind == kLambertian && use_nee) {  // directLight, render.zig:46-56
+      LightSample ls;
+      if (scene.lights.sample(hit.p, rng, &ls)) {
+        const double cos_s = dot(hit.n, ls.dir);
+        Hit hh;
+        if (cos_s > 0 && ls.pdf > 0 &&
+            !scene.intersect(Ray{hit.p, ls.dir}, kRayEpsilon, ls.distance * 0.999999, &hh, nullptr)) {
+          const double bsdf_pdf = cos_s / M_PI;
+          const double sc = cos_s / (M_PI * ls.pdf) * powerWeight(ls.pdf, bsdf_pdf);
+          const V3 al = scale(mul(mat.color, ls.radiance), sc);
+          radiance.v = add(radiance.v, mul(throughput.v, al));
+          for (int k = 0; k < np; ++k) radiance.d[k] = add(radiance.d[k], mul(throughput.d[k], al));
+          const int offa = g.offsets[hit.material];
+          if (offa >= 0)
+            for (int c = 0; c < 3; ++c)
+              radiance.d[offa + c] = add(radiance.d[offa + c],
+                                         mul(throughput.v, scale(mul(unitChan(c, 1.0), ls.radiance), sc)));
+          const int offl = g.offsets[scene.lights.items[ls.index].shape.material];
+          if (offl >= 0)
+            for (int c = 0; c < 3; ++c)
+              radiance.d[offl + c] = add(radiance.d[offl + c],
+                                         mul(throughput.v, scale(mul(mat.color, unitChan(c, 1.0)), sc)));
+        }
+      }
+    }
+    V3 att;
+    Ray next;
+    if (!scatter(mat, r, hit, rng, &att, &next)) break;
+    tmp = throughput;
+    throughput.v = mul(tmp.v, att);
+    for (int k = 0; k < np; ++k) throughput.d[k] = mul(tmp.d[k], att);
+    if (mat.kind == kLambertian || mat.kind == kMetal) {
+      const int off = g.offsets[hit.material];
+      if (off >= 0)
+        for (int c = 0; c < 3; ++c) throughput.d[off + c] = add(throughput.d[off + c], mul(tmp.v, unitChan(c, 1.0)));
+    }
+    prev_pdf = (use_nee && mat.kind == kLambertian) ? std::fmax(dot(next.d, hit.n), 0.0) / M_PI : 0;
+    prev_origin = hit.p;
+    r = next;
+    if (depth + 1 >= kRouletteDepth) {
+      const double p = std::fmin(maxComponent(throughput.v), 0.95);
+      if (p <= 0 || rng->uniform() >= p) break;
+      throughput.v = vdiv(throughput.v, p);
+      const double ip = 1.0 / p;
+      for (int k = 0; k < np; ++k) throughput.d[k] = scale(throughput.d[k], ip);
+    }
+  }
+  *out = radiance;
+  return radiance.v;
 }
 
 struct RenderCtx {  // render.zig:104-148
@@ -1557,6 +1668,61 @@ int orc_render(void* sp, const OrcOptions* o, double* out, uint64_t pixel_stride
   }
   return 0;
 }
+// Renders like orc_render and returns dLoss/dparams = sum over pixels of
+// dimg . dpixel/dparams (dimg: width*height*3 adjoint of the image) for the
+// material-colour parameters given by `offsets` (per material, -1 = none).
+int orc_render_grad(void* sp, const OrcOptions* o, const int32_t* offsets, int nparams, const double* dimg,
+                    double* image, double* grad) {
+  const Scene* s = (const Scene*)sp;
+  if (o->width == 0 || o->height == 0 || o->spp == 0 || o->tile_size == 0) return 1;
+  if (nparams > kMaxGradParams) return 2;
+  const uint64_t W = o->width, H = o->height;
+  const Camera cam = Camera::init(s->camera, (double)W / (double)H);
+  const GradCtx g{offsets, nparams};
+  const uint64_t nthreads = std::max<uint64_t>(1, std::min<uint64_t>(o->threads, H));
+  std::vector<std::vector<double>> part(nthreads, std::vector<double>(nparams, 0.0));
+  std::atomic<uint64_t> next_row{0};
+  auto work = [&](uint64_t t) {
+    DV3* dv = new DV3;
+    for (;;) {
+      const uint64_t y = next_row.fetch_add(1);
+      if (y >= H) break;
+      for (uint64_t x = 0; x < W; ++x) {
+        const uint64_t pix = y * W + x;
+        Rng rng = Rng::forStream(o->seed, pix);
+        V3 sum = splat(0);
+        const double inv_spp = 1.0 / (double)o->spp;
+        for (uint64_t k = 0; k < o->spp; ++k) {
+          const double su = ((double)x + rng.uniform()) / (double)W;
+          const double tv = ((double)y + rng.uniform()) / (double)H;
+          const Ray cr = cam.ray(su, tv, &rng);
+          const V3 c = traceWithGrad(*s, cr, &rng, o->max_depth, o->light_sampling != 0, g, dv);
+          if (!isFinite(c)) continue;
+          sum = add(sum, c);
+          for (int q = 0; q < nparams; ++q)
+            part[t][q] += (dimg[3 * pix] * dv->d[q].x + dimg[3 * pix + 1] * dv->d[q].y + dimg[3 * pix + 2] * dv->d[q].z) *
+                          inv_spp;
+        }
+        const V3 px = scale(sum, inv_spp);
+        image[3 * pix] = px.x;
+        image[3 * pix + 1] = px.y;
+        image[3 * pix + 2] = px.z;
+      }
+    }
+    delete dv;
+  };
+  std::vector<std::thread> pool;
+  for (uint64_t t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (int q = 0; q < nparams; ++q) {
+    double acc = 0;
+    for (uint64_t t = 0; t < nthreads; ++t) acc += part[t][q];
+    grad[q] = acc;
+  }
+  return 0;
+}
+
 uint64_t orc_hardware_threads() { return std::max(1u, std::thread::hardware_concurrency()); }
 
 }  // extern "C"

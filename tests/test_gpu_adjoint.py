@@ -1,0 +1,115 @@
+"""The path-replay adjoint (zt_gpu_backward) against
+  * the oracle's FP64 forward-mode (dual-number) gradient of the same
+    estimator, same streams: the device fp64 path to 1e-9, the fp32 product
+    path to 2e-3 (rounding and rare decision flips);
+  * central finite differences of the device forward with common random
+    numbers: exactly (1e-3) when no Russian roulette can occur (depth <= 3),
+    and within 1% in gradient norm at depth 8 once the roulette decision
+    flips that FD picks up are averaged out (256x256, 4096 spp, h = 1e-2);
+and the Adam driver on the differentiable box."""
+import numpy as np
+import pytest
+
+import zigtrace_amd as zt
+from zigtrace_amd import _native as N
+from zigtrace_amd import workloads as W
+from zigtrace_amd.adjoint import DiffRenderer, inverse_render
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def box_params(dr):
+    p = dr.get_params().astype(np.float64)
+    # Per-channel-distinct values away from the shipped ones (no channel ties).
+    rng = np.random.default_rng(4)
+    return (p * rng.uniform(0.85, 1.15, p.size)).astype(np.float32)
+
+
+def oracle_grad(w, dr, params, res, spp, depth, dimg, seed):
+    a = w.scene.view_arrays()
+    mats = a["materials"].copy()
+    for m, off in enumerate(dr.offsets):
+        if off >= 0:
+            mats[m]["color"] = params[off:off + 3].astype(np.float64)
+    a["materials"] = mats
+    ref = O.OracleScene.from_view_arrays(a)
+    return O.render_grad(ref, res, res, spp, depth, dr.offsets, dimg, seed=seed)
+
+
+@pytest.mark.parametrize("depth", [3, 8])
+def test_fp64_adjoint_equals_oracle_dual_numbers(gpu_ctx, depth):
+    w = W.diff_box(width=48, height=48, spp=8, max_depth=depth)
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options, precision=N.PRECISION_FP64)
+    p = box_params(dr)
+    rng = np.random.default_rng(1)
+    dimg = rng.uniform(-1, 1, (48, 48, 3)).astype(np.float32)
+    img = dr.forward(p)
+    g = dr.backward(p, dimg)
+    ref_img, ref_g = oracle_grad(w, dr, p, 48, 8, depth, dimg.astype(np.float64), seed=0)
+    dr.release()
+    assert np.isclose(img, ref_img, rtol=1e-9, atol=1e-12).all(axis=2).mean() > 0.995
+    assert np.allclose(g, ref_g, rtol=1e-7, atol=1e-9 * np.abs(ref_g).max())
+
+
+def test_fp32_adjoint_close_to_oracle(gpu_ctx):
+    w = W.diff_box(width=64, height=64, spp=16, max_depth=8)
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options)
+    p = box_params(dr)
+    dimg = np.ones((64, 64, 3), np.float32)
+    g = dr.backward(p, dimg)
+    _, ref_g = oracle_grad(w, dr, p, 64, 16, 8, dimg.astype(np.float64), seed=0)
+    dr.release()
+    assert np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g) < 2e-3
+
+
+def fd_grad(dr, p, o, dimg, h):
+    g = np.zeros(p.size)
+    for i in range(p.size):
+        up, dn = p.copy(), p.copy()
+        up[i] += h
+        dn[i] -= h
+        g[i] = (np.sum(dimg * dr.forward(up, o).astype(np.float64)) -
+                np.sum(dimg * dr.forward(dn, o).astype(np.float64))) / (2 * h)
+    return g
+
+
+def test_adjoint_equals_central_fd_without_roulette(gpu_ctx):
+    w = W.diff_box(width=64, height=64, spp=16, max_depth=3)
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options)
+    p = box_params(dr)
+    dimg = np.ones((64, 64, 3))
+    g = dr.backward(p, dimg.astype(np.float32))
+    fd = fd_grad(dr, p, w.options, dimg, 1e-3 * np.maximum(p, 1.0).astype(np.float32).min())
+    dr.release()
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-3
+
+
+def test_adjoint_within_1pct_of_central_fd_at_depth_8(gpu_ctx):
+    w = W.diff_box(width=256, height=256, spp=4096, max_depth=8)
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options)
+    p = box_params(dr)
+    dimg = np.ones((256, 256, 3))
+    g = dr.backward(p, dimg.astype(np.float32))
+    fd = fd_grad(dr, p, w.options, dimg, 1e-2)
+    dr.release()
+    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+    assert rel < 0.01, rel
+
+
+def test_inverse_rendering_recovers_parameters(gpu_ctx):
+    w = W.diff_box()  # 256x256, 32 spp, depth 8
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options)
+    truth = dr.get_params().astype(np.float64)
+    o = w.options
+    target = dr.forward(truth.astype(np.float32), zt.Options(o.width, o.height, 1024, o.max_depth, seed=777))
+    rng = np.random.default_rng(30)
+    init = truth * rng.uniform(0.7, 1.3, truth.size)
+    res = inverse_render(dr, target.astype(np.float64), init, steps=200, lr=1e-2)
+    dr.release()
+    assert res.losses[-1] < 0.2 * res.losses[0]
+    err0 = np.abs(init - truth) / truth
+    err1 = np.abs(res.params - truth) / truth
+    assert np.median(err1) < 0.5 * np.median(err0)
+    print(f"adam: {res.iters_per_s:.1f} fwd+bwd iters/s, loss {res.losses[0]:.4g} -> {res.losses[-1]:.4g}, "
+          f"median rel param error {np.median(err0):.3f} -> {np.median(err1):.3f}")

@@ -1,0 +1,49 @@
+"""The wavefront renderer produces the megakernel's image bit for bit, for
+any path-pool size (slot reuse across pixels), block size and precision."""
+import numpy as np
+import pytest
+
+import zigtrace_amd as zt
+from zigtrace_amd import _native as N
+
+from .conftest import scene_text
+
+pytestmark = pytest.mark.gpu
+
+MEGA = N.METHOD_MEGAKERNEL
+WAVE = N.METHOD_WAVEFRONT
+
+
+@pytest.mark.parametrize("name,w,h,spp,depth", [
+    ("cornell", 40, 40, 8, 16), ("spheres", 60, 34, 4, 16), ("glass", 48, 27, 6, 16), ("lamps", 48, 27, 6, 12),
+])
+@pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
+def test_wavefront_equals_megakernel(name, w, h, spp, depth, precision):
+    s = zt.parse(scene_text(name))
+    o = zt.Options(w, h, spp, depth, seed=5)
+    mega = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision))
+    for block, queue in [(256, 0), (64, 64), (128, 1000), (192, 333)]:
+        wave = zt.render(s, o, zt.Tuning(method=WAVE, precision=precision, block_size=block, queue_paths=queue))
+        assert np.array_equal(mega, wave), (name, block, queue)
+
+
+def test_wavefront_max_depth_zero_and_single_sample():
+    s = zt.parse(scene_text("cornell"))
+    for depth, spp in [(0, 3), (5, 1)]:
+        o = zt.Options(17, 9, spp, depth, seed=1)
+        mega = zt.render(s, o, zt.Tuning(method=MEGA))
+        wave = zt.render(s, o, zt.Tuning(method=WAVE, queue_paths=50))
+        assert np.array_equal(mega, wave)
+        if depth == 0:
+            assert not wave.any()
+
+
+def test_wavefront_no_nee_and_unbuilt_scene():
+    s = zt.parse(scene_text("lamps"))
+    o = zt.Options(32, 18, 4, 12, light_sampling=False)
+    assert np.array_equal(zt.render(s, o, zt.Tuning(method=MEGA)), zt.render(s, o, zt.Tuning(method=WAVE)))
+    u = zt.Scene()
+    m = u.add_material(zt.lambertian((0.5, 0.5, 0.5)))
+    u.add_sphere((0, 0, -3), 1, m)  # not built: brute-force path, no lights
+    o = zt.Options(16, 16, 4, 8)
+    assert np.array_equal(zt.render(u, o, zt.Tuning(method=MEGA)), zt.render(u, o, zt.Tuning(method=WAVE)))

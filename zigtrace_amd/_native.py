@@ -104,11 +104,20 @@ _SIGS = {
     "zt_gpu_scene_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_gpu_scene_release": (C.c_int, [C.c_void_p, C.c_uint32]),
     "zt_gpu_render": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "zt_gpu_render_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int]),
     "zt_gpu_trace_rays": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                     C.c_uint32, C.c_uint32, C.c_void_p]),
     "zt_gpu_intersect": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_int,
                                    C.c_uint32, C.c_void_p]),
     "zt_gpu_rng": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_void_p]),
+    "zt_gpu_param_count": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "zt_gpu_param_offsets": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "zt_gpu_get_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "zt_gpu_set_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "zt_gpu_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "zt_gpu_backward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.c_void_p]),
     # zt_host.h
     "zt_scene_create": (C.c_void_p, []),
     "zt_scene_destroy": (None, [C.c_void_p]),

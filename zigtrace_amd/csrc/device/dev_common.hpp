@@ -196,6 +196,8 @@ struct Light {
   R select;      // weight / total
   int kind;      // kKindSphere / kKindTriangle
   int prim;      // device primitive slot of the emitter
+  int material;  // emissive material of the emitter
+  int _pad;
 };
 
 // Camera after Camera.init (camera.zig:29-55), always f64: ray generation is

@@ -127,6 +127,20 @@ struct LightSample {
   V3<R> radiance;
   R pdf;
   int prim;
+  int material;
+};
+
+// Receives the contributions a path adds to its radiance, for the adjoint
+// (path replay). Rendering uses NoObserver, whose empty hooks vanish.
+struct NoObserver {
+  template <typename R>
+  ZT_D void miss(V3<R>) {}
+  template <typename R>
+  ZT_D void emission(V3<R>, V3<R>, int) {}
+  ZT_D void scattered(int) {}
+  template <typename R>
+  ZT_D void neePending(V3<R>, V3<R>, int) {}
+  ZT_D void neeCommitted() {}
 };
 
 // Lights.pick + Lights.sample (light.zig:106-166). The pick is done on the
@@ -154,7 +168,7 @@ ZT_D bool sampleLight(const SceneView<R>& sc, V3<R> origin, Rng& rng, LightSampl
     const V3<R> dir = scale(to, R(1) / dist);
     const R cos_l = -dot(dir, L.normal);
     if (cos_l <= R(1e-9)) return false;
-    *out = {dir, dist, L.radiance, select * dist * dist / (cos_l * L.area), L.prim};
+    *out = {dir, dist, L.radiance, select * dist * dist / (cos_l * L.area), L.prim, L.material};
     return true;
   }
   if (!outside(L.p0, L.radius, origin)) return false;
@@ -196,7 +210,7 @@ ZT_D bool sampleLight(const SceneView<R>& sc, V3<R> origin, Rng& rng, LightSampl
     t = t / len2(dir);
     if (!(t > 0)) return false;
   }
-  *out = {dir, t, L.radiance, select / (K<R>::two_pi * omc), L.prim};
+  *out = {dir, t, L.radiance, select / (K<R>::two_pi * omc), L.prim, L.material};
   return true;
 }
 
@@ -277,10 +291,12 @@ ZT_D int beginPath(const SceneView<R>& sc, const Params& pr, uint32_t x, uint32_
 // One bounce after the extension ray's query (render.zig:73-99). Returns the
 // next phase: kShadow when a next-event ray is pending, kExtend to continue,
 // kFinish when the path is complete.
-template <typename R>
-ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, Path<R>& st) {
+template <typename R, typename Obs = NoObserver>
+ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, Path<R>& st, Obs* obs = nullptr) {
   if (h.prim < 0) {
-    st.radiance = add(st.radiance, mul(st.throughput, background(sc, st.rd)));
+    const V3<R> x = mul(st.throughput, background(sc, st.rd));
+    st.radiance = add(st.radiance, x);
+    if (obs) obs->miss(x);
     return kFinish;
   }
   const Surface<R> s = surfaceAt(sc, st.ro, st.rd, h);
@@ -288,7 +304,9 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
   const bool emissive_front = m.kind == ZT_EMISSIVE && s.front;
   if (emissive_front && maxComponent(m.color) > 0) {
     const R w = st.prev_pdf > 0 ? powerWeight(st.prev_pdf, pdfHit(sc, st.prev_origin, s.p, h.prim)) : R(1);
-    st.radiance = add(st.radiance, scale(mul(st.throughput, m.color), w));
+    const V3<R> x = scale(mul(st.throughput, m.color), w);
+    st.radiance = add(st.radiance, x);
+    if (obs) obs->emission(x, scale(st.throughput, w), s.material);
   }
   bool pending = false;
   const bool lambertian = m.kind == ZT_LAMBERTIAN;
@@ -304,6 +322,7 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
         st.sh_tmax = ls.distance * R(0.999999);
         st.sh_excl = ls.prim;
         pending = true;
+        if (obs) obs->neePending(st.sh_contrib, scale(mul(st.throughput, m.color), sc_), ls.material);
       }
     }
   }
@@ -313,6 +332,7 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
   st.self = h.prim;
   bool alive = scattered;
   if (scattered) {
+    if (obs && (m.kind == ZT_LAMBERTIAN || m.kind == ZT_METAL)) obs->scattered(s.material);
     st.throughput = mul(st.throughput, att);
     st.prev_pdf = (pr.use_nee && lambertian) ? fmax(dot(dir, s.n), R(0)) / K<R>::pi : R(0);
     st.prev_origin = s.p;
@@ -332,9 +352,12 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
 
 // The shadow ray's outcome (render.zig:51): add the contribution if nothing
 // blocks the segment, then continue or finish.
-template <typename R>
-ZT_D int afterShadow(HitRec<R> h, Path<R>& st) {
-  if (h.prim < 0) st.radiance = add(st.radiance, st.sh_contrib);
+template <typename R, typename Obs = NoObserver>
+ZT_D int afterShadow(HitRec<R> h, Path<R>& st, Obs* obs = nullptr) {
+  if (h.prim < 0) {
+    st.radiance = add(st.radiance, st.sh_contrib);
+    if (obs) obs->neeCommitted();
+  }
   return st.terminated ? kFinish : kExtend;
 }
 

@@ -66,6 +66,12 @@ struct DevBuf {
   }
 };
 
+// A bounded emissive shape that Lights.build would consider, with the area
+// factor of its selection weight (weight = luminance(radiance) * weight_area).
+struct LightCandidate : ztd::Light<double> {
+  double weight_area;
+};
+
 // One uploaded scene: the device layout in both precisions plus what the
 // host needs to rebuild per-render constants.
 struct DevScene {
@@ -79,8 +85,16 @@ struct DevScene {
   // Host copies used to rebuild material/light tables when parameters change.
   std::vector<zt_material> materials;
   std::vector<ztd::Light<double>> lights_host;
-  std::vector<double> light_area_lum;  // per light: area (weight = luminance(Le) * area)
-  std::vector<int> light_material;     // per light: material id of the emitter
+  std::vector<LightCandidate> light_candidates;  // in shape order
+  // Differentiable parameters: 3 per Lambertian / metal / emissive material.
+  std::vector<int> param_offset;  // per material, -1 for dielectrics
+  DevBuf param_offsets;
+  int num_params = 0;
+};
+
+// Grow-only buffers of the wavefront renderer, reused across calls.
+struct WfBuffers {
+  DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, counters;
 };
 
 }  // namespace ztg
@@ -93,6 +107,7 @@ struct zt_ctx {
   std::vector<std::unique_ptr<ztg::DevScene>> scenes;  // index = scene id; null = released
   zt_gpu_stats stats{};
   ztg::DevBuf out, errflag, scratch;
+  ztg::WfBuffers wf;
 };
 
 namespace ztg {

@@ -11,6 +11,7 @@ void checkOptions(const zt_render_options* o);
 void resetErr(zt_ctx* ctx);
 void checkErr(zt_ctx* ctx);
 void uploadMaterialsAndLights(DevScene& s, hipStream_t stream);
+void rebuildLights(DevScene& s, hipStream_t stream);
 
 template <typename R>
 void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout);

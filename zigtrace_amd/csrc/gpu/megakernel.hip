@@ -189,7 +189,8 @@ void checkOptions(const zt_render_options* o) {
 
 template <typename R>
 void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout) {
-  const SceneView<R> view = viewOf<R>(s);
+  SceneView<R> view = viewOf<R>(s);
+  view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);  // render.zig:157,162
   Params pr = makeParams(s, o);
   uint32_t tw = tu.tile_w ? tu.tile_w : 8;
   uint32_t th = tu.tile_h ? tu.tile_h : 32;

@@ -247,6 +247,8 @@ Light<R> lightCast(const Light<double>& L) {
   o.select = (R)L.select;
   o.kind = L.kind;
   o.prim = L.prim;
+  o.material = L.material;
+  o._pad = 0;
   return o;
 }
 
@@ -455,10 +457,44 @@ int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id)
         L.p0 = vec(sh.v);
         L.radius = sh.v[3];
       }
+      L.material = (int)sh.material;
       s->lights_host.push_back(L);
-      s->light_material.push_back((int)sh.material);
       prim_light[L.prim] = (int)i;
     }
+    // Every bounded emissive shape, for rebuilding the light table when
+    // material colours change (light.zig:68-94).
+    for (uint64_t i = 0; i < n; ++i) {
+      const zt_shape& sh = v->shapes[i];
+      if (v->materials[sh.material].kind != ZT_EMISSIVE) continue;
+      LightCandidate c{};
+      c.prim = (int)slot_of[i];
+      c.material = (int)sh.material;
+      if (sh.kind == ZT_TRIANGLE) {
+        c.kind = (int)kKindTriangle;
+        c.p0 = vec(sh.v);
+        c.p1 = vec(sh.v + 3);
+        c.p2 = vec(sh.v + 6);
+        const V3<double> x = cross(sub(c.p1, c.p0), sub(c.p2, c.p0));
+        c.normal = normalize(x);
+        c.area = 0.5 * length(x);
+        c.weight_area = c.area;
+      } else {
+        c.kind = (int)kKindSphere;
+        c.p0 = vec(sh.v);
+        c.radius = sh.v[3];
+        c.weight_area = c.radius > 0 ? 4 * M_PI * c.radius * c.radius : 0;
+      }
+      s->light_candidates.push_back(c);
+    }
+    s->param_offset.assign(v->num_materials, -1);
+    for (uint64_t m = 0; m < v->num_materials; ++m)
+      if (v->materials[m].kind != ZT_DIELECTRIC) {
+        s->param_offset[m] = s->num_params;
+        s->num_params += 3;
+      }
+    std::vector<int> po(s->param_offset);
+    if (po.empty()) po.push_back(-1);
+    s->param_offsets.upload(po, ctx->stream);
     std::vector<double> cdf(v->light_cdf, v->light_cdf + v->num_lights);
     s->cdf.upload(cdf, ctx->stream);
     s->prim_light.upload(prim_light, ctx->stream);

@@ -1,12 +1,427 @@
-// Wavefront renderer (under construction).
+// Wavefront renderer: the bounce loop split into three persistent kernels
+// over compacted queues of path slots, captured in a hipGraph.
+//
+//   extend  : closest-hit query of every queued extension ray
+//   shade   : one bounce per queued path (render.zig:73-99); appends the path
+//             to the shadow queue if a next-event ray is pending, to the next
+//             extension queue if it continues, or finishes it
+//   shadow  : any-hit query of every queued next-event ray (render.zig:51)
+//
+// A bounded pool of path slots works through all pixels of all scenes in the
+// batch: a slot owns one pixel at a time (its xoshiro256++ stream and
+// accumulator), runs that pixel's samples one after the other as the
+// reference does (render.zig:133-144), then claims the next unstarted pixel
+// from a global counter. Every path runs the same device functions as the
+// megakernel (beginPath / shade / afterShadow / traverse), so the image is
+// bit-identical to it for any pool size, block size or queue order.
+//
+// Queue appends are compacted per wave: a ballot gives each lane its rank
+// and one atomic per wave reserves the wave's range.
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../device/dev_path.hpp"
+#include "ctx.hpp"
 #include "launch.hpp"
+#include "wavefront.hpp"
+
+using namespace ztd;
 
 namespace ztg {
+
+static constexpr double kRayEps = 1e-4;  // render.zig:26
+
 template <typename R>
-void launchWavefront(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&, R*) {
-  throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront method not available");
+struct WfShared {
+  WfScene<R>* scenes;
+  WfSlot<R>* slots;
+  float2* hits;       // per slot: (t, prim bits) of the last extension query
+  double* hits_d;     // per slot: t in f64 for the fp64 path
+  uint32_t* qext[2];  // extension queues (double-buffered)
+  uint32_t* qsh;      // shadow queue
+  uint32_t* counters; // [0..1] qext counts, [2] qsh count, [3] next pixel, [4] error flag
+  uint32_t num_scenes;
+  uint32_t num_slots;
+  uint64_t total_pixels;
+};
+
+// Wave-compacted append of `slot` to queue q (count at *cnt) when pred holds.
+ZT_D void append(uint32_t* q, uint32_t* cnt, uint32_t slot, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  if (pred) q[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = slot;
 }
-template void launchWavefront<float>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&, float*);
+
+template <typename R>
+ZT_D uint32_t findScene(const WfShared<R>& w, uint64_t p) {
+  uint32_t lo = 0, hi = w.num_scenes - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (w.scenes[mid].first_pixel <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Starts global pixel p in slot `sl` (its stream, accumulator and first
+// sample). Returns false only if the pixel completed at once (max_depth == 0).
+template <typename R>
+ZT_D bool startPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
+  const uint32_t si = findScene(w, p);
+  const WfScene<R>& sc = w.scenes[si];
+  const uint32_t local = (uint32_t)(p - sc.first_pixel);
+  sl.scene = si;
+  sl.pixel = local;
+  sl.sample = 0;
+  sl.sum = splat(R(0));
+  sl.rng = rngForStream(sc.pr.seed, local);  // render.zig:135
+  const uint32_t x = local % sc.pr.width, y = local / sc.pr.width;
+  if (beginPath<R>(sc.view, sc.pr, x, y, sl.rng, sl.p) == kExtend) return true;
+  // max_depth == 0: every sample is empty and the pixel is zero (render.zig:72).
+  sc.out[3 * (uint64_t)local + 0] = R(0);
+  sc.out[3 * (uint64_t)local + 1] = R(0);
+  sc.out[3 * (uint64_t)local + 2] = R(0);
+  return false;
+}
+
+// Claims pixels from the global counter until one has a ray to trace.
+template <typename R>
+ZT_D bool claimPixel(const WfShared<R>& w, WfSlot<R>& sl) {
+  for (;;) {
+    const uint64_t p = atomicAdd((unsigned long long*)&w.counters[6], 1ull);
+    if (p >= w.total_pixels) {
+      sl.scene = 0xFFFFFFFFu;
+      return false;
+    }
+    if (startPixel(w, sl, p)) return true;
+  }
+}
+
+// A path of the slot ended: accumulate it (render.zig:141-142) and start the
+// pixel's next sample, or write the pixel (render.zig:144) and claim another.
+// Returns true if the slot has an extension ray to trace.
+template <typename R>
+ZT_D bool finishPath(const WfShared<R>& w, WfSlot<R>& sl) {
+  const WfScene<R>& sc = w.scenes[sl.scene];
+  if (isFinite(sl.p.radiance)) sl.sum = add(sl.sum, sl.p.radiance);
+  if (++sl.sample < sc.pr.spp) {
+    const uint32_t x = sl.pixel % sc.pr.width, y = sl.pixel / sc.pr.width;
+    beginPath<R>(sc.view, sc.pr, x, y, sl.rng, sl.p);  // max_depth > 0 here: always an extension ray
+    return true;
+  }
+  const R inv_spp = R(1) / R(sc.pr.spp);
+  sc.out[3 * (uint64_t)sl.pixel + 0] = sl.sum.x * inv_spp;
+  sc.out[3 * (uint64_t)sl.pixel + 1] = sl.sum.y * inv_spp;
+  sc.out[3 * (uint64_t)sl.pixel + 2] = sl.sum.z * inv_spp;
+  return claimPixel(w, sl);
+}
+
+template <typename R>
+__global__ void wfInit(WfShared<R> w) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_slots; s += stride) {
+    WfSlot<R> sl;
+    bool go = false;
+    sl.scene = 0xFFFFFFFFu;
+    if (s < w.total_pixels) go = startPixel(w, sl, s) || claimPixel(w, sl);
+    w.slots[s] = sl;
+    append(w.qext[0], &w.counters[0], s, go);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void wfExtend(WfShared<R> w, int cur) {
+  extern __shared__ int lds[];
+  const LaneStack stk{lds + threadIdx.x, (float*)(lds + kLds * blockDim.x) + threadIdx.x, (int)blockDim.x};
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.counters[1 - cur] = 0;  // next extension queue
+    w.counters[2] = 0;        // shadow queue
+  }
+  const uint32_t n = w.counters[cur];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  int overflow = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t s = w.qext[cur][i];
+    const WfSlot<R>& sl = w.slots[s];
+    const SceneView<R>& v = w.scenes[sl.scene].view;
+    const V3<R> o = sl.p.ro, d = sl.p.rd;
+    const HitRec<R> h = (v.num_nodes == 0 && v.num_prims > 0)
+                            ? bruteForce<R>(v, o, d, R(kRayEps), R(INFINITY), sl.p.self, -1, false)
+                            : traverse<R>(v, stk, o, d, R(kRayEps), R(INFINITY), sl.p.self, -1, false, &overflow);
+    if constexpr (K<R>::exact) w.hits_d[s] = h.t;
+    else w.hits[s].x = h.t;
+    w.hits[s].y = __int_as_float(h.prim);
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
+  const uint32_t n = w.counters[cur];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const int nxt = 1 - cur;
+  // Uniform trip count across the wave so every lane reaches the appends.
+  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  for (uint32_t b = base0; b < n; b += stride) {
+    const uint32_t i = b + (threadIdx.x & 63u);
+    const bool active = i < n;
+    uint32_t s = 0;
+    bool to_sh = false, to_ext = false;
+    if (active) {
+      s = w.qext[cur][i];
+      WfSlot<R> sl = w.slots[s];
+      const WfScene<R>& sc = w.scenes[sl.scene];
+      HitRec<R> h;
+      if constexpr (K<R>::exact) h.t = w.hits_d[s];
+      else h.t = w.hits[s].x;
+      h.prim = __float_as_int(w.hits[s].y);
+      const int ph = shade<R>(sc.view, sc.pr, h, sl.rng, sl.p);
+      if (ph == kShadow) to_sh = true;
+      else if (ph == kExtend) to_ext = true;
+      else to_ext = finishPath(w, sl);
+      if (ph == kShadow && !sl.p.terminated) to_ext = true;
+      w.slots[s] = sl;
+    }
+    append(w.qsh, &w.counters[2], s, to_sh);
+    append(w.qext[nxt], &w.counters[nxt], s, to_ext);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void wfShadow(WfShared<R> w, int nxt) {
+  extern __shared__ int lds[];
+  const LaneStack stk{lds + threadIdx.x, (float*)(lds + kLds * blockDim.x) + threadIdx.x, (int)blockDim.x};
+  const uint32_t n = w.counters[2];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  int overflow = 0;
+  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  for (uint32_t b = base0; b < n; b += stride) {
+    const uint32_t i = b + (threadIdx.x & 63u);
+    const bool active = i < n;
+    uint32_t s = 0;
+    bool to_ext = false;
+    if (active) {
+      s = w.qsh[i];
+      WfSlot<R> sl = w.slots[s];
+      const SceneView<R>& v = w.scenes[sl.scene].view;
+      const HitRec<R> h = (v.num_nodes == 0 && v.num_prims > 0)
+                              ? bruteForce<R>(v, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
+                                              sl.p.sh_excl, true)
+                              : traverse<R>(v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
+                                            sl.p.sh_excl, true, &overflow);
+      if (afterShadow<R>(h, sl.p) == kFinish) to_ext = finishPath(w, sl);
+      w.slots[s] = sl;
+    }
+    append(w.qext[nxt], &w.counters[nxt], s, to_ext);
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+__global__ void wfReset(uint32_t* counters, uint64_t first_free_pixel) {
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 6; ++i) counters[i] = 0;
+    *(unsigned long long*)&counters[6] = first_free_pixel;
+  }
+}
+
+static size_t wfLds(uint32_t threads) { return (size_t)2 * kLds * threads * sizeof(int); }
+
+template <typename R>
+static int gridFor(const void* kernel, uint32_t threads, size_t lds) {
+  int dev = 0, cus = 0, per = 0;
+  ZT_HIP(hipGetDevice(&dev));
+  ZT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  ZT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)threads, lds));
+  return std::max(1, cus * std::max(per, 1));
+}
+
+template <typename R>
+void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_gpu_tuning& tu) {
+  uint64_t total = 0;
+  for (const auto& s : scenes) total += (uint64_t)s.pr.width * s.pr.height;
+  if (total == 0) return;
+  uint32_t threads = tu.block_size ? tu.block_size : 256;
+  if (threads < 64 || threads > 256 || threads % 64)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront block size must be 64, 128, 192 or 256");
+  uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(total, 1ull << 22);
+  const uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
+
+  WfBuffers& b = ctx->wf;
+  const size_t slot_bytes = sizeof(WfSlot<R>);
+  b.scenes.ensure(scenes.size() * sizeof(WfScene<R>));
+  b.slots.ensure((size_t)nslots * slot_bytes);
+  b.hits.ensure((size_t)nslots * sizeof(float2));
+  b.hits_d.ensure((size_t)nslots * sizeof(double));
+  b.qext0.ensure((size_t)nslots * 4);
+  b.qext1.ensure((size_t)nslots * 4);
+  b.qsh.ensure((size_t)nslots * 4);
+  b.counters.ensure(64);
+  ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
+                        ctx->stream));
+
+  WfShared<R> w{};
+  w.scenes = b.scenes.as<WfScene<R>>();
+  w.slots = b.slots.as<WfSlot<R>>();
+  w.hits = b.hits.as<float2>();
+  w.hits_d = b.hits_d.as<double>();
+  w.qext[0] = b.qext0.as<uint32_t>();
+  w.qext[1] = b.qext1.as<uint32_t>();
+  w.qsh = b.qsh.as<uint32_t>();
+  w.counters = b.counters.as<uint32_t>();
+  w.num_scenes = (uint32_t)scenes.size();
+  w.num_slots = nslots;
+  w.total_pixels = total;
+
+  const size_t lds = wfLds(threads);
+  const int g_ext = gridFor<R>((const void*)wfExtend<R>, threads, lds);
+  const int g_sh = gridFor<R>((const void*)wfShadow<R>, threads, lds);
+  const int g_shade = gridFor<R>((const void*)wfShade<R>, threads, 0);
+  const int g_init = std::max(1, (int)std::min<uint64_t>((nslots + threads - 1) / threads, 65535));
+
+  hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, ctx->stream, w.counters, (uint64_t)nslots);
+  hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, ctx->stream, w);
+  ZT_HIP(hipGetLastError());
+  ctx->stats.launches += 2;
+
+  // Bounce iterations in batches of an even count, captured once as a graph.
+  const int kBatch = 16;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  for (int j = 0; j < kBatch; ++j) {
+    const int cur = j & 1;
+    hipLaunchKernelGGL(wfExtend<R>, dim3(g_ext), dim3(threads), lds, ctx->stream, w, cur);
+    hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
+    hipLaunchKernelGGL(wfShadow<R>, dim3(g_sh), dim3(threads), lds, ctx->stream, w, 1 - cur);
+  }
+  const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (ce != hipSuccess) throw Error(ZT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    throw Error(ZT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  }
+  uint32_t active = 1;
+  uint64_t batches = 0;
+  // Each batch advances every live path by kBatch bounces; a frame needs at
+  // most spp * (max_depth + 1) of them per pixel pass, so the loop is bounded.
+  uint64_t max_batches = 1;
+  for (const auto& s : scenes) {
+    const uint64_t per_pixel = (uint64_t)s.pr.spp * ((uint64_t)std::min<uint32_t>(s.pr.max_depth, 1u << 20) + 2) * 2;
+    const uint64_t rounds = (total + nslots - 1) / nslots;
+    max_batches = std::max<uint64_t>(max_batches, per_pixel * rounds / kBatch + 2);
+  }
+  hipError_t le = hipSuccess;
+  while (active) {
+    le = hipGraphLaunch(exec, ctx->stream);
+    if (le != hipSuccess) break;
+    ++batches;
+    le = hipMemcpyAsync(&active, w.counters, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (le != hipSuccess) break;
+    le = hipStreamSynchronize(ctx->stream);
+    if (le != hipSuccess) break;
+    if (batches > max_batches) {
+      le = hipErrorUnknown;
+      break;
+    }
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (le != hipSuccess) {
+    if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
+    throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
+  }
+  ctx->stats.launches += batches * kBatch * 3;
+  uint32_t flag = 0;
+  ZT_HIP(hipMemcpyAsync(&flag, w.counters + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  ZT_HIP(hipStreamSynchronize(ctx->stream));
+  if (flag) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+}
+template void runWavefront<float>(zt_ctx*, const std::vector<WfScene<float>>&, const zt_gpu_tuning&);
+template void runWavefront<double>(zt_ctx*, const std::vector<WfScene<double>>&, const zt_gpu_tuning&);
+
+template <typename R>
+void launchWavefront(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout) {
+  WfScene<R> ws{};
+  ws.view = viewOf<R>(s);
+  ws.pr = makeParams(s, o);
+  ws.view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
+  ws.out = dout;
+  ws.first_pixel = 0;
+  runWavefront<R>(ctx, std::vector<WfScene<R>>{ws}, tu);
+}
+template void launchWavefront<float>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
+                                     float*);
 template void launchWavefront<double>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
                                       double*);
+
 }  // namespace ztg
+
+using namespace ztg;
+
+extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, const zt_render_options* opts,
+                                   const zt_gpu_tuning* tuning, void* const* outs, int out_on_device) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (n == 0) return;
+    if (!scene_ids || !opts || !outs) throw Error(ZT_ERR_INVALID_ARGUMENT, "null argument");
+    zt_gpu_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.precision > ZT_PRECISION_FP64) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad precision");
+    for (uint32_t i = 0; i < n; ++i) {
+      checkOptions(&opts[i]);
+      if (!outs[i]) throw Error(ZT_ERR_INVALID_ARGUMENT, "null output");
+      sceneOf(ctx, scene_ids[i]);
+    }
+    setDevice(ctx);
+    const bool f64 = tu.precision == ZT_PRECISION_FP64;
+    const size_t elem = f64 ? sizeof(double) : sizeof(float);
+    std::vector<size_t> bytes(n), offs(n);
+    size_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      bytes[i] = (size_t)opts[i].width * opts[i].height * 3 * elem;
+      offs[i] = total;
+      total += (bytes[i] + 255) & ~(size_t)255;
+    }
+    if (!out_on_device) ctx->out.ensure(total);
+    auto devOut = [&](uint32_t i) -> void* {
+      return out_on_device ? outs[i] : (void*)((char*)ctx->out.p + offs[i]);
+    };
+    ctx->stats.launches = 0;
+    ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    uint64_t first = 0, paths = 0;
+    auto build = [&](auto tag) {
+      using R = decltype(tag);
+      std::vector<WfScene<R>> scenes(n);
+      for (uint32_t i = 0; i < n; ++i) {
+        const DevScene& s = sceneOf(ctx, scene_ids[i]);
+        scenes[i].view = viewOf<R>(s);
+        scenes[i].view.cam = cameraInit(s.camera, (double)opts[i].width / (double)opts[i].height);
+        scenes[i].pr = makeParams(s, opts[i]);
+        scenes[i].out = (R*)devOut(i);
+        scenes[i].first_pixel = first;
+        first += (uint64_t)opts[i].width * opts[i].height;
+        paths += (uint64_t)opts[i].width * opts[i].height * opts[i].spp;
+      }
+      runWavefront<R>(ctx, scenes, tu);
+    };
+    if (f64) build(double{});
+    else build(float{});
+    ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!out_on_device)
+      for (uint32_t i = 0; i < n; ++i)
+        ZT_HIP(hipMemcpyAsync(outs[i], devOut(i), bytes[i], hipMemcpyDeviceToHost, ctx->stream));
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    ZT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.kernel_ms = ms;
+    ctx->stats.paths = paths;
+  });
+}

@@ -132,6 +132,27 @@ class Context:
         self._check(N.lib().zt_gpu_render(self._h, scene_id, N.ptr(opts.record()), N.ptr(t.record()),
                                           C.c_void_p(device_ptr), 1))
 
+    def render_batch(self, scene_ids, options, tuning: Tuning | None = None, outs=None):
+        """Renders several scenes in one wavefront pass; returns host images."""
+        t = tuning or Tuning(method=N.METHOD_WAVEFRONT)
+        dt = np.float64 if t.precision == N.PRECISION_FP64 else np.float32
+        n = len(scene_ids)
+        if outs is None:
+            outs = [np.zeros((o.height, o.width, 3), dtype=dt) for o in options]
+        ids = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+        recs = np.concatenate([o.record() for o in options])
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        self._check(N.lib().zt_gpu_render_batch(self._h, n, N.ptr(ids), N.ptr(recs), N.ptr(t.record()), ptrs, 0))
+        return outs
+
+    def render_batch_device(self, scene_ids, options, tuning: Tuning | None, device_ptrs) -> None:
+        t = tuning or Tuning(method=N.METHOD_WAVEFRONT)
+        n = len(scene_ids)
+        ids = np.ascontiguousarray(scene_ids, dtype=np.uint32)
+        recs = np.concatenate([o.record() for o in options])
+        ptrs = (C.c_void_p * n)(*device_ptrs)
+        self._check(N.lib().zt_gpu_render_batch(self._h, n, N.ptr(ids), N.ptr(recs), N.ptr(t.record()), ptrs, 1))
+
     def trace_rays(self, scene_id: int, rays: np.ndarray, states: np.ndarray, max_depth: int,
                    light_sampling: bool = True, precision: int = N.PRECISION_FP32) -> np.ndarray:
         """traceWith per ray; `states` ((n, 4) uint64) is advanced in place."""

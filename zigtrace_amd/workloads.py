@@ -213,3 +213,62 @@ def diff_box(albedos=None, radiance=None, spp: int = 32, max_depth: int = 8, wid
     s.build()
     return Workload("diff_box", s, Options(width, height, spp, max_depth),
                     "Cornell-style box, 10 triangles + 2 spheres, 8 albedo materials + 1 emitter")
+
+
+def batch_scenes(count: int = 256, seed: int = 11, min_prims: int = 100, max_prims: int = 200_000,
+                 resolutions=(64, 128, 256, 512), spps=(4, 16, 64), depth_range=(8, 32)) -> list:
+    """Independent scenes of uneven size and settings for the batched renderer: primitive
+    counts log-uniform in [min_prims, max_prims], half sphere fields over a ground plane and
+    half triangle soups, mixed Lambertian/metal/dielectric materials, 1-4 small emissive
+    spheres plus a sky; square resolution, spp and depth drawn per scene."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(min_prims), np.log(max_prims)))))
+        res = int(rng.choice(resolutions))
+        spp = int(rng.choice(spps))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        s = Scene()
+        mats = []
+        for _ in range(12):
+            u = rng.random()
+            if u < 0.6:
+                mats.append(s.add_material(lambertian(tuple(0.1 + 0.8 * rng.random(3)))))
+            elif u < 0.85:
+                mats.append(s.add_material(metal(tuple(0.5 + 0.5 * rng.random(3)), 0.4 * rng.random())))
+            else:
+                mats.append(s.add_material(dielectric(1.3 + 0.4 * rng.random())))
+        mats = np.array(mats)
+        half = 10.0
+        kind = "spheres" if rng.random() < 0.5 else "soup"
+        arr = _shapes(n)
+        arr["material"] = mats[rng.integers(0, len(mats), n)]
+        if kind == "spheres":
+            ground = s.add_material(lambertian((0.5, 0.5, 0.5)))
+            s.add_plane((0, -half, 0), (0, 1, 0), ground)
+            r = 0.35 * half / n ** (1 / 3)
+            arr["kind"] = N.SPHERE
+            arr["v"][:, :3] = rng.uniform(-half, half, (n, 3))
+            arr["v"][:, 3] = r * rng.uniform(0.5, 1.5, n)
+        else:
+            edge = 1.5 * half / n ** (1 / 3)
+            c = rng.uniform(-half, half, (n, 3))
+            arr["kind"] = N.TRIANGLE
+            arr["v"] = np.concatenate([c + rng.uniform(-edge, edge, (n, 3)) for _ in range(3)], axis=1)
+        nl = int(rng.integers(1, 5))
+        lights = _shapes(nl)
+        for k in range(nl):
+            le = 5 + 45 * rng.random()
+            m = s.add_material(emissive((le, le, le * (0.6 + 0.4 * rng.random()))))
+            lights[k]["kind"] = N.SPHERE
+            lights[k]["material"] = m
+            lights[k]["v"][:4] = list(rng.uniform(-half, half, 3)) + [0.3 + 0.3 * rng.random()]
+        s.add_many(np.concatenate([arr, lights]))
+        eye = rng.normal(size=3)
+        eye = eye / np.linalg.norm(eye) * half * 3.2
+        s.camera = CameraConfig(tuple(eye), (0, 0, 0), (0, 1, 0), 40.0)
+        s.background = sky((0.6, 0.6, 0.6), (0.2, 0.3, 0.6))
+        s.build()
+        out.append(Workload(f"batch{i}_{kind}", s, Options(res, res, spp, depth, seed=i),
+                            f"{kind}, {n} primitives, {res}x{res}, {spp} spp, depth {depth}"))
+    return out
