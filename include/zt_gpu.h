@@ -62,9 +62,10 @@ typedef struct zt_gpu_tuning {
   uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
   uint32_t tile_w;      /* megakernel: pixels per workgroup row (0 = default) */
   uint32_t tile_h;      /* megakernel: pixel rows per workgroup (0 = default) */
-  uint32_t block_size;  /* wavefront: threads per workgroup (0 = default) */
-  uint32_t queue_paths; /* wavefront: paths in flight per pass (0 = every pixel) */
-  uint32_t _reserved[2];
+  uint32_t block_size;    /* wavefront: threads per workgroup (0 = default) */
+  uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
+  uint32_t variant;       /* megakernel: 0 default, 1..3 = compiled for >= 4/5/6 waves per SIMD */
+  uint32_t stack_entries; /* traversal stack entries kept in LDS per lane (0 = 16; even, 8..64) */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

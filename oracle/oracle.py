@@ -89,7 +89,8 @@ def lib() -> C.CDLL:
 
 
 def _p(a: np.ndarray) -> C.c_void_p:
-    return C.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array, so temporaries outlive the call.
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def _rec(dtype, **kw) -> np.ndarray:

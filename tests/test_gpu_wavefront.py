@@ -47,3 +47,30 @@ def test_wavefront_no_nee_and_unbuilt_scene():
     u.add_sphere((0, 0, -3), 1, m)  # not built: brute-force path, no lights
     o = zt.Options(16, 16, 4, 8)
     assert np.array_equal(zt.render(u, o, zt.Tuning(method=MEGA)), zt.render(u, o, zt.Tuning(method=WAVE)))
+
+
+@pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
+def test_kernel_variants_and_stack_sizes_are_bit_identical(precision):
+    """Occupancy variants and LDS stack capacities change only where traversal
+    entries live (registers, LDS or private spill), never the visiting order."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=2, width=64, height=36)
+    s, o = wl.scene, wl.options
+    base = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision))
+    for variant in range(4):
+        for cap in (8, 16, 32, 64):
+            img = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision, variant=variant, stack_entries=cap))
+            assert np.array_equal(base, img), (variant, cap)
+    for cap in (8, 24, 64):
+        img = zt.render(s, o, zt.Tuning(method=WAVE, precision=precision, stack_entries=cap, queue_paths=700))
+        assert np.array_equal(base, img), cap
+
+
+def test_rejects_bad_tuning():
+    s = zt.parse(scene_text("cornell"))
+    o = zt.Options(8, 8, 1, 4)
+    for tu in (zt.Tuning(method=MEGA, stack_entries=7), zt.Tuning(method=MEGA, stack_entries=66),
+               zt.Tuning(method=MEGA, variant=9), zt.Tuning(method=WAVE, stack_entries=5)):
+        with pytest.raises(zt.ZTError) as e:
+            zt.render(s, o, tu)
+        assert e.value.name == "InvalidArgument"

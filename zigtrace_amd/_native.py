@@ -51,7 +51,7 @@ RAY = np.dtype([("origin", "<f8", (3,)), ("dir", "<f8", (3,))])
 HIT = np.dtype([("t", "<f8"), ("point", "<f8", (3,)), ("normal", "<f8", (3,)), ("front_face", "<u4"),
                 ("material", "<u4"), ("hit", "<i4"), ("prim", "<i4")])
 TUNING = np.dtype([("method", "<u4"), ("precision", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"),
-                   ("block_size", "<u4"), ("queue_paths", "<u4"), ("_reserved", "<u4", (2,))])
+                   ("block_size", "<u4"), ("queue_paths", "<u4"), ("variant", "<u4"), ("stack_entries", "<u4")])
 STATS = np.dtype([("kernel_ms", "<f8"), ("upload_ms", "<f8"), ("launches", "<u8"), ("paths", "<u8"),
                   ("extend_rays", "<u8"), ("shadow_rays", "<u8")])
 SCENE_INFO = np.dtype([("num_materials", "<u8"), ("num_shapes", "<u8"), ("num_planes", "<u8"), ("num_nodes", "<u8"),
@@ -170,7 +170,8 @@ def declared_symbols() -> list[str]:
 
 
 def ptr(a: np.ndarray | None) -> C.c_void_p | None:
-    return None if a is None else C.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array, so temporaries outlive the call.
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def check(code: int, ctx=None) -> None:

@@ -129,21 +129,28 @@ ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
   return !(hi < lo);
 }
 
-// Per-lane traversal stack: the top kLds entries live in LDS (laid out
+// Per-lane traversal stack: the top `cap` entries live in LDS (laid out
 // [entry][lane], 4-byte words, so a wave's accesses are bank-conflict free
 // whatever its lanes' depths), the rest spill in halves to private memory.
-static constexpr int kLds = 16;
-static constexpr int kHalf = kLds / 2;
-static constexpr int kSpill = 64;
-// Deepest device tree the stack can traverse (checked at upload): a BVH
-// needs at most depth + 1 entries with nearest-first pushes.
-static constexpr int kMaxTreeDepth = kLds + kSpill - 1;
+static constexpr int kLds = 16;  // default LDS entries per lane
+static constexpr int kSpill = 80;
+// Deepest device tree the stack can traverse for the smallest LDS capacity
+// used (8 entries), checked at upload: a BVH needs at most depth + 1
+// entries with nearest-first pushes.
+static constexpr int kMaxTreeDepth = 8 + kSpill - 1;
 
 struct LaneStack {
   int* code;    // LDS: code[i * stride]
   float* tval;  // LDS: tval[i * stride]
   int stride;
+  int cap;      // LDS entries per lane (even, >= 8)
 };
+
+// LDS bytes of a stack of `cap` entries per lane for `threads` lanes.
+ZT_HD size_t stackBytes(int cap, int threads) { return (size_t)2 * cap * threads * sizeof(int); }
+ZT_D LaneStack laneStack(int* lds, int cap) {
+  return LaneStack{lds + threadIdx.x, (float*)(lds + cap * blockDim.x) + threadIdx.x, (int)blockDim.x, cap};
+}
 
 template <typename R>
 ZT_D float entryT(R t) {
@@ -171,20 +178,21 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
     stk.code[0] = 0;
     stk.tval[0] = entryT<R>(tmin);
     sp = 1;
+    const int half = stk.cap >> 1;
     auto push = [&](int c, R t) {
-      if (sp == kLds) {
-        if (nsp + kHalf > kSpill) {
+      if (sp == stk.cap) {
+        if (nsp + half > kSpill) {
           *overflow = 1;
           return;
         }
-        for (int i = 0; i < kHalf; ++i) {
+        for (int i = 0; i < half; ++i) {
           spill_code[nsp + i] = stk.code[i * stk.stride];
           spill_t[nsp + i] = stk.tval[i * stk.stride];
-          stk.code[i * stk.stride] = stk.code[(i + kHalf) * stk.stride];
-          stk.tval[i * stk.stride] = stk.tval[(i + kHalf) * stk.stride];
+          stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
+          stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
         }
-        nsp += kHalf;
-        sp = kHalf;
+        nsp += half;
+        sp = half;
       }
       stk.code[sp * stk.stride] = c;
       stk.tval[sp * stk.stride] = entryT<R>(t);
@@ -193,12 +201,12 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
     for (;;) {
       if (sp == 0) {
         if (nsp == 0) break;
-        nsp -= kHalf;
-        for (int i = 0; i < kHalf; ++i) {
+        nsp -= half;
+        for (int i = 0; i < half; ++i) {
           stk.code[i * stk.stride] = spill_code[nsp + i];
           stk.tval[i * stk.stride] = spill_t[nsp + i];
         }
-        sp = kHalf;
+        sp = half;
       }
       --sp;
       const int code = stk.code[sp * stk.stride];
@@ -261,6 +269,156 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
   }
   return HitRec<R>{closest, best};
 }
+
+// Resumable form of `traverse` for kernels whose lanes take a new ray as
+// soon as theirs is finished (dynamic ray fetch). It performs the same
+// sequence of stack operations, box tests and primitive tests as `traverse`,
+// split into an interior phase (`inner`, called until a leaf is popped or the
+// stack is empty) and a leaf phase (`leafStep`), so the hit is identical.
+template <typename R>
+struct Walker {
+  V3<R> o, d, inv;
+  R tmin, closest;
+  int best, self, excl;
+  int sp, nsp;
+  int leaf;  // popped leaf code waiting for leafStep, or kNoChild
+  bool nx, ny, nz, anyhit;
+  bool bvh_done;  // no BVH entries left (or an any-hit query found one)
+  bool found;     // any-hit query hit something
+  int spill_code[kSpill];
+  float spill_t[kSpill];
+
+  ZT_D void start(const SceneView<R>& sc, const LaneStack& stk, V3<R> o_, V3<R> d_, R tmin_, R tmax, int self_,
+                  int excl_, bool anyhit_) {
+    o = o_;
+    d = d_;
+    tmin = tmin_;
+    closest = tmax;
+    best = -1;
+    self = self_;
+    excl = excl_;
+    anyhit = anyhit_;
+    found = false;
+    leaf = kNoChild;
+    nsp = 0;
+    inv = mk(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+    nx = inv.x < 0;
+    ny = inv.y < 0;
+    nz = inv.z < 0;
+    if (sc.num_nodes > 0) {
+      stk.code[0] = 0;
+      stk.tval[0] = entryT<R>(tmin);
+      sp = 1;
+      bvh_done = false;
+    } else {
+      sp = 0;
+      bvh_done = true;
+    }
+  }
+  ZT_D void push(const LaneStack& stk, int c, R t, int* overflow) {
+    const int half = stk.cap >> 1;
+    if (sp == stk.cap) {
+      if (nsp + half > kSpill) {
+        *overflow = 1;
+        return;
+      }
+      for (int i = 0; i < half; ++i) {
+        spill_code[nsp + i] = stk.code[i * stk.stride];
+        spill_t[nsp + i] = stk.tval[i * stk.stride];
+        stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
+        stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
+      }
+      nsp += half;
+      sp = half;
+    }
+    stk.code[sp * stk.stride] = c;
+    stk.tval[sp * stk.stride] = entryT<R>(t);
+    ++sp;
+  }
+  // One pop: an interior node is traversed, a leaf is parked in `leaf`.
+  ZT_D void inner(const SceneView<R>& sc, const LaneStack& stk, int* overflow) {
+    if (sp == 0) {
+      if (nsp == 0) {
+        bvh_done = true;
+        return;
+      }
+      const int half = stk.cap >> 1;
+      nsp -= half;
+      for (int i = 0; i < half; ++i) {
+        stk.code[i * stk.stride] = spill_code[nsp + i];
+        stk.tval[i * stk.stride] = spill_t[nsp + i];
+      }
+      sp = half;
+    }
+    --sp;
+    const int code = stk.code[sp * stk.stride];
+    const float et = stk.tval[sp * stk.stride];
+    if (R(et) > closest) return;
+    if (code < 0) {
+      leaf = code;
+      return;
+    }
+    const DNode n = sc.nodes[code];
+    R tl, tr;
+    const bool hl =
+        slab<R>(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, o, inv, nx, ny, nz, tmin, closest, &tl) && n.d.x != kNoChild;
+    const bool hr =
+        slab<R>(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, o, inv, nx, ny, nz, tmin, closest, &tr) && n.d.y != kNoChild;
+    if (hl && hr) {
+      const bool near_left = tl <= tr;
+      push(stk, near_left ? n.d.y : n.d.x, near_left ? tr : tl, overflow);
+      push(stk, near_left ? n.d.x : n.d.y, near_left ? tl : tr, overflow);
+    } else if (hl) {
+      push(stk, n.d.x, tl, overflow);
+    } else if (hr) {
+      push(stk, n.d.y, tr, overflow);
+    }
+  }
+  ZT_D void leafStep(const SceneView<R>& sc) {
+    constexpr bool kIdentity = !K<R>::exact;
+    const int v = -1 - leaf;
+    leaf = kNoChild;
+    const int first = v >> 4, count = v & 15;
+    for (int k = first; k < first + count; ++k) {
+      if (kIdentity && k == excl) continue;
+      const Prim<R> p = loadPrim<R>(sc, k);
+      R t;
+      bool h;
+      if (metaKind(p.meta) == kKindTriangle) {
+        h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
+      } else {
+        h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
+                                     : sphereHit<R>(p, o, d, tmin, closest, &t);
+      }
+      if (h) {
+        closest = t;
+        best = k;
+        if (anyhit) {
+          found = true;
+          bvh_done = true;
+          return;
+        }
+      }
+    }
+  }
+  // Planes after the BVH (scene.zig:113-114), then the result.
+  ZT_D HitRec<R> finish(const SceneView<R>& sc) {
+    constexpr bool kIdentity = !K<R>::exact;
+    if (!found) {
+      for (int j = 0; j < sc.num_planes; ++j) {
+        const int id = sc.num_prims + j;
+        if (kIdentity && (id == self || id == excl)) continue;
+        R t;
+        if (planeHit<R>(sc.planes[j], o, d, tmin, closest, &t)) {
+          closest = t;
+          best = id;
+          if (anyhit) break;
+        }
+      }
+    }
+    return HitRec<R>{closest, best};
+  }
+};
 
 // Brute force over every primitive in slot order, then planes
 // (geometry.zig:129-139 and scene.zig:109-117): the reference the BVH

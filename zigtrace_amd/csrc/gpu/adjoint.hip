@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void adjointKernel(SceneView<R> sc, Params pr,
                                                      uint32_t ntiles, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const LaneStack stk = laneStack(lds, kLds);
   R* lg = (R*)(lds + 2 * kLds * blockDim.x);
   for (int i = (int)tid; i < nparams; i += (int)blockDim.x) lg[i] = R(0);
   __syncthreads();
@@ -264,7 +264,7 @@ void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const f
   const uint32_t tw = 8, threads = 64;
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;
   const uint32_t tiles_y = (pr.height + (threads / tw) - 1) / (threads / tw);
-  const size_t lds = (size_t)2 * kLds * threads * sizeof(int) + (size_t)s.num_params * sizeof(R);
+  const size_t lds = stackBytes(kLds, (int)threads) + (size_t)s.num_params * sizeof(R);
   hipLaunchKernelGGL((adjointKernel<R, K>), dim3(tiles_x * tiles_y), dim3(threads), lds, ctx->stream, view, pr, ddimg,
                      s.param_offsets.as<int>(), s.num_params, dgrad, tw, tiles_x, tiles_x * tiles_y,
                      ctx->errflag.as<int>());

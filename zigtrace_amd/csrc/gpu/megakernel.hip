@@ -37,11 +37,11 @@ ZT_D uint32_t xcdTile(uint32_t b, uint32_t nblocks) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void megakernel(SceneView<R> sc, Params pr, R* __restrict__ out, uint32_t tile_w,
-                                                  uint32_t tile_h, uint32_t tiles_x, uint32_t ntiles, int* err) {
+ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict__ out, uint32_t tile_w,
+                         uint32_t tile_h, uint32_t tiles_x, uint32_t ntiles, int cap, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const LaneStack stk = laneStack(lds, cap);
   const uint32_t tile = xcdTile(blockIdx.x, ntiles);
   const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
   const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
@@ -77,13 +77,23 @@ __global__ __launch_bounds__(256) void megakernel(SceneView<R> sc, Params pr, R*
   if (overflow) atomicOr(err, 1);
 }
 
+// The same kernel compiled for different occupancy targets: W > 0 asks the
+// compiler to fit at least W waves per SIMD (registers beyond that budget
+// spill). Every variant renders the same bits.
+template <typename R, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1, 8))) void megakernel(
+    SceneView<R> sc, Params pr, R* __restrict__ out, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
+    uint32_t ntiles, int cap, int* err) {
+  megakernelBody<R>(sc, pr, out, tile_w, tile_h, tiles_x, ntiles, cap, err);
+}
+
 // traceWith for caller-given rays and RNG states (render.zig:61-102).
 template <typename R>
 __global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, const zt_ray* rays, uint64_t n,
                                                    uint64_t* states, double* out, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const LaneStack stk = laneStack(lds, kLds);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
   if (i >= n) return;
   Rng rng{states[4 * i], states[4 * i + 1], states[4 * i + 2], states[4 * i + 3]};
@@ -123,7 +133,7 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
                                                        int mode, zt_hit* out, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk{lds + tid, (float*)(lds + kLds * blockDim.x) + tid, (int)blockDim.x};
+  const LaneStack stk = laneStack(lds, kLds);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
   if (i >= n) return;
   const V3<R> o = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
@@ -164,7 +174,7 @@ __global__ void rngKernel(uint64_t seed, const uint64_t* streams, uint64_t n, ui
   }
 }
 
-static size_t ldsBytes(uint32_t threads) { return (size_t)2 * kLds * threads * sizeof(int); }
+static size_t ldsBytes(uint32_t threads) { return stackBytes(kLds, (int)threads); }
 
 Params makeParams(const DevScene& s, const zt_render_options& o) {
   Params p{};
@@ -200,8 +210,30 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;
   const uint32_t tiles_y = (pr.height + th - 1) / th;
   const uint32_t ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL(megakernel<R>, dim3(ntiles), dim3(threads), ldsBytes(threads), ctx->stream, view, pr, dout, tw,
-                     th, tiles_x, ntiles, ctx->errflag.as<int>());
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : kLds;
+  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  const size_t lds = stackBytes(cap, (int)threads);
+  int* err = ctx->errflag.as<int>();
+  switch (tu.variant) {
+    case 0:
+      hipLaunchKernelGGL((megakernel<R, 0>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, err);
+      break;
+    case 1:
+      hipLaunchKernelGGL((megakernel<R, 4>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, err);
+      break;
+    case 2:
+      hipLaunchKernelGGL((megakernel<R, 5>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, err);
+      break;
+    case 3:
+      hipLaunchKernelGGL((megakernel<R, 6>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, err);
+      break;
+    default:
+      throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..3");
+  }
   ZT_HIP(hipGetLastError());
   ctx->stats.launches += 1;
 }

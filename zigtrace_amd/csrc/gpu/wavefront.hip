@@ -41,7 +41,9 @@ struct WfShared {
   double* hits_d;     // per slot: t in f64 for the fp64 path
   uint32_t* qext[2];  // extension queues (double-buffered)
   uint32_t* qsh;      // shadow queue
-  uint32_t* counters; // [0..1] qext counts, [2] qsh count, [3] next pixel, [4] error flag
+  // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64),
+  // [8] extension fetch counter, [9] shadow fetch counter
+  uint32_t* counters;
   uint32_t num_scenes;
   uint32_t num_slots;
   uint64_t total_pixels;
@@ -136,28 +138,77 @@ __global__ void wfInit(WfShared<R> w) {
   }
 }
 
+// Closest-hit (extension queue) or any-hit (shadow queue) queries with
+// dynamic ray fetch: a lane whose ray is finished takes the next queue entry
+// at once (one atomic per wave per refill), so waves stay full while their
+// rays need different numbers of steps. Interior nodes are walked until every
+// busy lane holds a leaf, then leaves are tested (while-while).
 template <typename R>
-__global__ __launch_bounds__(256) void wfExtend(WfShared<R> w, int cur) {
+__global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shadow, int cap) {
   extern __shared__ int lds[];
-  const LaneStack stk{lds + threadIdx.x, (float*)(lds + kLds * blockDim.x) + threadIdx.x, (int)blockDim.x};
+  const LaneStack stk = laneStack(lds, cap);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    w.counters[1 - cur] = 0;  // next extension queue
-    w.counters[2] = 0;        // shadow queue
+    if (shadow) {
+      w.counters[8] = 0;  // extension fetch counter of the next iteration
+    } else {
+      w.counters[1 - cur] = 0;  // next extension queue
+      w.counters[2] = 0;        // shadow queue
+      w.counters[9] = 0;        // shadow fetch counter
+    }
   }
-  const uint32_t n = w.counters[cur];
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t n = shadow ? w.counters[2] : w.counters[cur];
+  const uint32_t* q = shadow ? w.qsh : w.qext[cur];
+  uint32_t* fetch = &w.counters[shadow ? 9 : 8];
+  const uint32_t lane = __lane_id();
+  Walker<R> wk;
+  const SceneView<R>* v = nullptr;
+  uint32_t slot = 0;
+  bool has = false, exhausted = false;
   int overflow = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t s = w.qext[cur][i];
-    const WfSlot<R>& sl = w.slots[s];
-    const SceneView<R>& v = w.scenes[sl.scene].view;
-    const V3<R> o = sl.p.ro, d = sl.p.rd;
-    const HitRec<R> h = (v.num_nodes == 0 && v.num_prims > 0)
-                            ? bruteForce<R>(v, o, d, R(kRayEps), R(INFINITY), sl.p.self, -1, false)
-                            : traverse<R>(v, stk, o, d, R(kRayEps), R(INFINITY), sl.p.self, -1, false, &overflow);
-    if constexpr (K<R>::exact) w.hits_d[s] = h.t;
-    else w.hits[s].x = h.t;
-    w.hits[s].y = __int_as_float(h.prim);
+  for (;;) {
+    const uint64_t need = __ballot(!has && !exhausted);
+    if (need) {
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(need));
+      base = __shfl(base, (int)leader);
+      if (!has) {
+        const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (idx < n) {
+          slot = q[idx];
+          const WfSlot<R>& sl = w.slots[slot];
+          v = &w.scenes[sl.scene].view;
+          if (shadow) wk.start(*v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self, sl.p.sh_excl, true);
+          else wk.start(*v, stk, sl.p.ro, sl.p.rd, R(kRayEps), R(INFINITY), sl.p.self, -1, false);
+          has = true;
+          if (v->num_nodes == 0 && v->num_prims > 0) {  // no BVH: brute force (scene.zig:109-112)
+            const HitRec<R> h = bruteForce<R>(*v, wk.o, wk.d, wk.tmin, wk.closest, wk.self, wk.excl, wk.anyhit);
+            wk.closest = h.t;
+            wk.best = h.prim;
+            wk.found = true;  // final: bruteForce already tested the planes
+            wk.bvh_done = true;
+          }
+        }
+      }
+      if (base + (uint32_t)__popcll(need) >= n) exhausted = true;
+    }
+    if (__ballot(has) == 0) break;
+    for (;;) {
+      if (!(has && !wk.bvh_done && wk.leaf == kNoChild)) break;
+      wk.inner(*v, stk, &overflow);
+    }
+    if (has && wk.leaf != kNoChild) wk.leafStep(*v);
+    if (has && wk.bvh_done) {
+      // found: an any-hit query stopped early, or bruteForce gave the answer.
+      const HitRec<R> h = wk.found ? HitRec<R>{wk.closest, wk.best} : wk.finish(*v);
+      if constexpr (K<R>::exact) {
+        if (!shadow) w.hits_d[slot] = h.t;
+      } else {
+        if (!shadow) w.hits[slot].x = h.t;
+      }
+      w.hits[slot].y = __int_as_float(h.prim);
+      has = false;
+    }
   }
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
@@ -194,44 +245,34 @@ __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
   }
 }
 
+// Applies each shadow ray's outcome (render.zig:51) and finishes the paths
+// that ended at that bounce.
 template <typename R>
-__global__ __launch_bounds__(256) void wfShadow(WfShared<R> w, int nxt) {
-  extern __shared__ int lds[];
-  const LaneStack stk{lds + threadIdx.x, (float*)(lds + kLds * blockDim.x) + threadIdx.x, (int)blockDim.x};
+__global__ __launch_bounds__(256) void wfResolve(WfShared<R> w, int nxt) {
   const uint32_t n = w.counters[2];
   const uint32_t stride = gridDim.x * blockDim.x;
-  int overflow = 0;
   const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
   for (uint32_t b = base0; b < n; b += stride) {
     const uint32_t i = b + (threadIdx.x & 63u);
-    const bool active = i < n;
     uint32_t s = 0;
     bool to_ext = false;
-    if (active) {
+    if (i < n) {
       s = w.qsh[i];
       WfSlot<R> sl = w.slots[s];
-      const SceneView<R>& v = w.scenes[sl.scene].view;
-      const HitRec<R> h = (v.num_nodes == 0 && v.num_prims > 0)
-                              ? bruteForce<R>(v, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
-                                              sl.p.sh_excl, true)
-                              : traverse<R>(v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
-                                            sl.p.sh_excl, true, &overflow);
+      const HitRec<R> h{R(0), __float_as_int(w.hits[s].y)};
       if (afterShadow<R>(h, sl.p) == kFinish) to_ext = finishPath(w, sl);
       w.slots[s] = sl;
     }
     append(w.qext[nxt], &w.counters[nxt], s, to_ext);
   }
-  if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
 __global__ void wfReset(uint32_t* counters, uint64_t first_free_pixel) {
   if (threadIdx.x == 0) {
-    for (int i = 0; i < 6; ++i) counters[i] = 0;
+    for (int i = 0; i < 16; ++i) counters[i] = 0;
     *(unsigned long long*)&counters[6] = first_free_pixel;
   }
 }
-
-static size_t wfLds(uint32_t threads) { return (size_t)2 * kLds * threads * sizeof(int); }
 
 template <typename R>
 static int gridFor(const void* kernel, uint32_t threads, size_t lds) {
@@ -279,10 +320,12 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.num_slots = nslots;
   w.total_pixels = total;
 
-  const size_t lds = wfLds(threads);
-  const int g_ext = gridFor<R>((const void*)wfExtend<R>, threads, lds);
-  const int g_sh = gridFor<R>((const void*)wfShadow<R>, threads, lds);
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : kLds;
+  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  const size_t lds = stackBytes(cap, (int)threads);
+  const int g_tr = gridFor<R>((const void*)wfTrace<R>, threads, lds);
   const int g_shade = gridFor<R>((const void*)wfShade<R>, threads, 0);
+  const int g_res = gridFor<R>((const void*)wfResolve<R>, threads, 0);
   const int g_init = std::max(1, (int)std::min<uint64_t>((nslots + threads - 1) / threads, 65535));
 
   hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, ctx->stream, w.counters, (uint64_t)nslots);
@@ -297,9 +340,10 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   for (int j = 0; j < kBatch; ++j) {
     const int cur = j & 1;
-    hipLaunchKernelGGL(wfExtend<R>, dim3(g_ext), dim3(threads), lds, ctx->stream, w, cur);
+    hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, 0, cap);
     hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
-    hipLaunchKernelGGL(wfShadow<R>, dim3(g_sh), dim3(threads), lds, ctx->stream, w, 1 - cur);
+    hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, 1, cap);
+    hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, ctx->stream, w, 1 - cur);
   }
   const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
   if (ce != hipSuccess) throw Error(ZT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -338,7 +382,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
-  ctx->stats.launches += batches * kBatch * 3;
+  ctx->stats.launches += batches * kBatch * 4;
   uint32_t flag = 0;
   ZT_HIP(hipMemcpyAsync(&flag, w.counters + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   ZT_HIP(hipStreamSynchronize(ctx->stream));

@@ -43,10 +43,13 @@ class Tuning:
     tile_h: int = 0
     block_size: int = 0
     queue_paths: int = 0
+    variant: int = 0
+    stack_entries: int = 0
 
     def record(self) -> np.ndarray:
         return N.record(N.TUNING, method=self.method, precision=self.precision, tile_w=self.tile_w,
-                        tile_h=self.tile_h, block_size=self.block_size, queue_paths=self.queue_paths)
+                        tile_h=self.tile_h, block_size=self.block_size, queue_paths=self.queue_paths,
+                        variant=self.variant, stack_entries=self.stack_entries)
 
 
 def render(scene: Scene, opts: Options, tuning: Tuning | None = None) -> np.ndarray:
