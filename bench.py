@@ -4,19 +4,29 @@ Workload (one GPU): the triangle-heavy mesh field -- 64 displaced icospheres
 (subdivision 5, 1,310,720 triangles) over a Lambertian plane, lit by 64 small
 emissive spheres and a sky; depth 16, next-event estimation with MIS, fp32.
 A step is one full render of that frame (132,710,400 camera paths) into a
-framebuffer that stays in HBM; scene build and upload happen before the timed
-region.
+framebuffer that stays in HBM; scene build, upload and autotuning happen
+before the timed region.
 
-Also reported, not as `value`:
-  * per-method Mpaths/s (plain megakernel, wavefront when available),
-  * the roofline of the render kernel: algorithmic bytes per launch (BVH node
+Reported beside `value` (which is the fastest method's whole-frame rate):
+  * per-method Mpaths/s (plain megakernel, wavefront), each with the launch
+    configuration the autotuner picked,
+  * Mrays/s (extension + shadow rays, counted by the oracle on a sample),
+  * the roofline of the megakernel: algorithmic bytes per launch (BVH node
     and primitive fetches counted by the oracle on a sample of the same
     frame, 64 B/node, 48 B/triangle, 32 B/sphere, 16 B/plane test, plus the
-    12 B/pixel framebuffer write) over the launch's HIP-event duration,
+    12 B/pixel framebuffer write) over the launch's HIP-event duration on the
+    context stream the kernel runs on,
+  * batch: 256 seeded scenes of uneven size, resolution, spp and depth in one
+    persistent wavefront pass vs one megakernel launch per scene,
+  * adjoint: forward + path-replay backward iterations/s of Adam inverse
+    rendering on the 256x256, 32 spp differentiable box,
   * the CPU baseline: the oracle (C++ f64 restatement of the reference CPU
     path, tiles + atomic counter, all threads) on a strided pixel sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method auto|megakernel|wavefront]
+                       [--no-cpu] [--no-extra] [--no-tune] [--spp S]
+       python bench.py --copy-probe BYTES   (one device-to-device copy of BYTES, for
+                                             calibrating HBM traffic counters)
 """
 from __future__ import annotations
 
@@ -45,8 +55,7 @@ def algorithmic_bytes_per_path(ctr, spp: int) -> float:
 
 
 def cpu_baseline(workload, target_seconds: float = 15.0):
-    """The oracle on a strided pixel sample of the same frame: (Mpaths/s, threads, sample, counters)."""
-    import numpy as np
+    """The oracle on a strided pixel sample of the same frame: (record, counters, rays per path)."""
     from oracle import oracle as O
 
     o = workload.options
@@ -68,10 +77,83 @@ def cpu_baseline(workload, target_seconds: float = 15.0):
     paths = int(ctr["paths"])
     rays = int(ctr["extend_rays"]) + int(ctr["shadow_rays"])
     return {
-        "value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+        "value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
         "sample": f"every {stride}th pixel of the {o.width}x{o.height} frame, all {o.spp} spp ({paths} paths, "
                   f"{dt:.1f}s), C++ f64 restatement of the reference CPU path",
     }, ctr, rays / max(paths, 1)
+
+
+def copy_probe(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (plus one warm-up): the known-size kernel that HBM
+    traffic counters are scaled against."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copied {nbytes} bytes")
+    return 0
+
+
+def bench_batch(zt, N, W, steps: int):
+    """256 scenes: one persistent wavefront pass vs one megakernel launch per scene."""
+    import numpy as np
+    t0 = time.time()
+    wls = W.batch_scenes()
+    log(f"[bench] batch: {len(wls)} scenes built in {time.time() - t0:.1f}s")
+    ctx = zt.Context(0)
+    ids = [ctx.upload(w.scene) for w in wls]
+    opts = [w.options for w in wls]
+    paths = sum(o.width * o.height * o.spp for o in opts)
+    outs = [ctx.alloc(o.width * o.height * 12) for o in opts]
+    tu_w = zt.Tuning(method=N.METHOD_WAVEFRONT)
+    tu_m = zt.Tuning(method=N.METHOD_MEGAKERNEL)
+    ctx.render_batch_device(ids, opts, tu_w, outs)  # warm-up
+    t = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_batch_device(ids, opts, tu_w, outs)
+    wave_s = (time.perf_counter() - t) / steps
+    for sid, o, d in zip(ids, opts, outs):  # warm-up
+        ctx.render_device(sid, o, tu_m, d)
+    t = time.perf_counter()
+    for _ in range(steps):
+        for sid, o, d in zip(ids, opts, outs):
+            ctx.render_device(sid, o, tu_m, d)
+    mega_s = (time.perf_counter() - t) / steps
+    for d in outs:
+        ctx.free(d)
+    ctx.close()
+    return {"scenes": len(wls), "paths": paths, "steps": steps,
+            "persistent_wavefront_mpaths_s": round(paths / wave_s / 1e6, 2),
+            "per_scene_megakernel_mpaths_s": round(paths / mega_s / 1e6, 2),
+            "persistent_over_per_scene": round(mega_s / wave_s, 3),
+            "prims_total": int(sum(w.scene.info()["num_shapes"] for w in wls))}
+
+
+def bench_adjoint(zt, N, W, steps: int):
+    """Adam inverse rendering on the differentiable box: forward + backward per iteration."""
+    import numpy as np
+    from zigtrace_amd.adjoint import DiffRenderer, inverse_render
+    wl = W.diff_box()
+    ctx = zt.Context(0)
+    dr = DiffRenderer(ctx, wl.scene, wl.options)
+    truth = dr.get_params()
+    rng = np.random.default_rng(5)
+    init = truth * rng.uniform(0.7, 1.3, truth.shape)
+    o = wl.options
+    target = dr.forward(truth, zt.Options(o.width, o.height, 1024, o.max_depth, seed=999)).astype(np.float64)
+    inverse_render(dr, target, init, steps=2, lr=1e-2)  # warm-up
+    res = inverse_render(dr, target, init, steps=steps, lr=1e-2)
+    err0 = float(np.abs(init - truth).mean() / np.abs(truth).mean())
+    err1 = float(np.abs(res.params - truth).mean() / np.abs(truth).mean())
+    dr.release()
+    ctx.close()
+    return {"iters_per_s": round(res.iters_per_s, 2), "steps": steps, "width": o.width, "height": o.height,
+            "spp": o.spp, "max_depth": o.max_depth, "params": int(truth.size),
+            "loss_first": res.losses[0], "loss_last": res.losses[-1],
+            "param_rel_err_init": round(err0, 4), "param_rel_err_final": round(err1, 4)}
 
 
 def main() -> int:
@@ -81,7 +163,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--method", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batch and adjoint legs")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner")
+    ap.add_argument("--spp", type=int, default=64, help="samples per pixel (64 is the benchmark; less for profiling)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the path-tracing core is benchmarked on a single "
                                    "MI355X (images shard by pixel rows across GPUs with no exchange; not run here)"}))
@@ -94,7 +182,7 @@ def main() -> int:
     from zigtrace_amd import workloads as W
 
     t0 = time.time()
-    w = W.mesh_field()
+    w = W.mesh_field(spp=args.spp)
     o = w.options
     log(f"[bench] workload {w.description}: built in {time.time() - t0:.1f}s")
     ctx = zt.Context(0)
@@ -109,13 +197,13 @@ def main() -> int:
     results = {}
     for name in chosen:
         tu = zt.Tuning(method=methods[name])
-        try:
-            for i in range(args.warmup):
-                ctx.render_device(sid, o, tu, dout)
-                log(f"[bench] {name} warmup {i + 1}/{args.warmup}: {ctx.stats()['kernel_ms']:.1f} ms")
-        except zt.ZTError as e:
-            log(f"[bench] {name} unavailable: {e}")
-            continue
+        if not args.no_tune:
+            t = time.time()
+            tu, probe_ms = ctx.autotune(sid, o, method=methods[name])
+            log(f"[bench] {name} autotuned in {time.time() - t:.1f}s: {tu} ({probe_ms:.1f} ms probe)")
+        for i in range(args.warmup):
+            ctx.render_device(sid, o, tu, dout)
+            log(f"[bench] {name} warmup {i + 1}/{args.warmup}: {ctx.stats()['kernel_ms']:.1f} ms")
         kms = []
         t = time.perf_counter()
         for i in range(args.steps):
@@ -124,10 +212,8 @@ def main() -> int:
             log(f"[bench] {name} step {i + 1}/{args.steps}: {kms[-1]:.1f} ms")
         wall = time.perf_counter() - t
         results[name] = {"ms_per_step": wall * 1e3 / args.steps, "kernel_ms": float(np.mean(kms)),
-                         "mpaths_s": paths * args.steps / wall / 1e6}
-    if not results:
-        log("[bench] no method ran")
-        return 1
+                         "mpaths_s": paths * args.steps / wall / 1e6,
+                         "tuning": {k: v for k, v in tu.__dict__.items() if v}}
     img = ctx.to_host(dout, np.zeros((o.height, o.width, 3), np.float32))
     assert np.isfinite(img).all()
     best = max(results, key=lambda k: results[k]["mpaths_s"])
@@ -138,12 +224,25 @@ def main() -> int:
         cpu, ctr, rays_per_path = cpu_baseline(w)
         log(f"[bench] cpu baseline {cpu['value']:.3f} Mpaths/s on {cpu['cores']} threads")
     roofline = None
-    if ctr is not None:
+    if ctr is not None and "megakernel" in results:
         bpp = algorithmic_bytes_per_path(ctr, o.spp)
-        achieved = paths * bpp / (r["kernel_ms"] * 1e-3) / 1e9
+        km = results["megakernel"]["kernel_ms"]
+        achieved = paths * bpp / (km * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                    "bytes_per_path": round(bpp, 2), "kernel": best}
+                    "kernel": "megakernel", "kernel_ms": round(km, 3), "bytes_per_path": round(bpp, 2),
+                    "bytes_per_launch": round(paths * bpp)}
+    extra = {}
+    if not args.no_extra:
+        ctx.free(dout)
+        dout = None
+        ctx.close()
+        t = time.time()
+        extra["batch"] = bench_batch(zt, N, W, steps=1)
+        log(f"[bench] batch leg {time.time() - t:.1f}s: {extra['batch']}")
+        t = time.time()
+        extra["adjoint"] = bench_adjoint(zt, N, W, steps=50)
+        log(f"[bench] adjoint leg {time.time() - t:.1f}s: {extra['adjoint']}")
     line = {
         "metric": "Mpaths/s @1080p 64spp (plain vs wavefront-batched); fwd+adjoint iters/s",
         "value": round(r["mpaths_s"], 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
@@ -153,12 +252,15 @@ def main() -> int:
                                "64 emissive spheres r=0.05, Lambertian plane, sky",
                    "width": o.width, "height": o.height, "spp": o.spp, "max_depth": o.max_depth,
                    "light_sampling": True, "method": best},
-        "methods": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in results.items()},
+        "methods": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in results.items()},
         "mrays_s": round(r["mpaths_s"] * rays_per_path, 2) if rays_per_path else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        **extra,
     }
-    ctx.free(dout)
+    if dout is not None:
+        ctx.free(dout)
     print(json.dumps(line), flush=True)
     return 0
 

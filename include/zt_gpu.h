@@ -54,18 +54,24 @@ typedef struct zt_scene_view {
 } zt_scene_view;
 
 /* GPU-only knobs. None of them changes the image: the same seed gives the
- * same bits for every value (tested). Zero means "default / tuned". */
+ * same bits for every value (tested). Zero means "default".
+ * ZT_METHOD_AUTO (or a NULL tuning) renders with the fastest configuration
+ * zt_gpu_autotune has recorded on this context for the scene's size class and
+ * the image size, and with the megakernel defaults when none is recorded; the
+ * other knobs of an AUTO tuning are then ignored (precision excepted). */
 enum { ZT_METHOD_AUTO = 0, ZT_METHOD_MEGAKERNEL = 1, ZT_METHOD_WAVEFRONT = 2 };
 enum { ZT_PRECISION_FP32 = 0, ZT_PRECISION_FP64 = 1 };
 typedef struct zt_gpu_tuning {
   uint32_t method;      /* ZT_METHOD_* */
   uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
-  uint32_t tile_w;      /* megakernel: pixels per workgroup row (0 = default) */
-  uint32_t tile_h;      /* megakernel: pixel rows per workgroup (0 = default) */
-  uint32_t block_size;    /* wavefront: threads per workgroup (0 = default) */
+  uint32_t tile_w;      /* megakernel: pixels per workgroup row (0 = 16) */
+  uint32_t tile_h;      /* megakernel: pixel rows per workgroup (0 = 16) */
+  uint32_t block_size;    /* wavefront: threads per workgroup (0 = 256) */
   uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
-  uint32_t variant;       /* megakernel: 0 default, 1..3 = compiled for >= 4/5/6 waves per SIMD */
-  uint32_t stack_entries; /* traversal stack entries kept in LDS per lane (0 = 16; even, 8..64) */
+  uint32_t variant;       /* megakernel build: 0 default (fp32: 4, fp64: 1), 1 = registers unconstrained,
+                             2/3/4 = compiled for >= 4/5/6 waves per SIMD */
+  uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64
+                             (0 = megakernel fp32: 8, fp64: 16; wavefront: 16) */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */
@@ -118,6 +124,18 @@ int zt_gpu_render(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts,
  * uneven size, spp and depth share the GPU without per-scene launches. */
 int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, const zt_render_options* opts,
                         const zt_gpu_tuning* tuning, void* const* outs, int out_on_device);
+
+/* Autotuner (no reference counterpart). Renders the scene at the options'
+ * resolution with min(spp, probe_spp) samples (probe_spp 0 = 4) under each
+ * candidate launch configuration of `method` (AUTO: both methods) --
+ * megakernel: build variant x LDS stack entries x tile shape; wavefront:
+ * block size x path-slot pool x LDS stack entries -- times them with HIP
+ * events, records the fastest per method in the context (keyed by method,
+ * precision, image size and the scene's primitive count class) and writes the
+ * overall fastest to *best and its time to *best_ms (either may be NULL).
+ * Later AUTO renders of matching scenes use the record. */
+int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, uint32_t method,
+                    uint32_t precision, uint32_t probe_spp, zt_gpu_tuning* best, double* best_ms);
 
 /* traceWith (render.zig:61-102) for n caller-given rays, each with its own
  * xoshiro256++ state (4 x u64 per ray, advanced in place as the reference

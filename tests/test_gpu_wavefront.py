@@ -57,7 +57,7 @@ def test_kernel_variants_and_stack_sizes_are_bit_identical(precision):
     wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=2, width=64, height=36)
     s, o = wl.scene, wl.options
     base = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision))
-    for variant in range(4):
+    for variant in range(5):
         for cap in (8, 16, 32, 64):
             img = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision, variant=variant, stack_entries=cap))
             assert np.array_equal(base, img), (variant, cap)
@@ -74,3 +74,24 @@ def test_rejects_bad_tuning():
         with pytest.raises(zt.ZTError) as e:
             zt.render(s, o, tu)
         assert e.value.name == "InvalidArgument"
+
+
+def test_autotune_records_a_configuration_with_identical_output():
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=1, meshes=9, spp=3, width=48, height=32)
+    ctx = zt.Context(0)
+    try:
+        sid = ctx.upload(wl.scene)
+        o = wl.options
+        default = ctx.render(sid, o, zt.Tuning(method=MEGA))
+        for method in (MEGA, WAVE, N.METHOD_AUTO):
+            tu, ms = ctx.autotune(sid, o, method=method, probe_spp=1)
+            assert ms > 0 and tu.method in (MEGA, WAVE)
+            if method != N.METHOD_AUTO:
+                assert tu.method == method
+            assert np.array_equal(ctx.render(sid, o, tu), default), (method, tu)
+        # AUTO (and no tuning at all) now renders with the record: same bits.
+        assert np.array_equal(ctx.render(sid, o, zt.Tuning()), default)
+        assert np.array_equal(ctx.render(sid, o, None), default)
+    finally:
+        ctx.close()

@@ -275,6 +275,14 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
 // sequence of stack operations, box tests and primitive tests as `traverse`,
 // split into an interior phase (`inner`, called until a leaf is popped or the
 // stack is empty) and a leaf phase (`leafStep`), so the hit is identical.
+// Overflow of a lane's LDS stack into private memory. Kept apart from the
+// Walker so the Walker's scalar state can live in registers (an aggregate
+// with a dynamically indexed array member is kept whole in private memory).
+struct SpillStack {
+  int code[kSpill];
+  float t[kSpill];
+};
+
 template <typename R>
 struct Walker {
   V3<R> o, d, inv;
@@ -285,8 +293,6 @@ struct Walker {
   bool nx, ny, nz, anyhit;
   bool bvh_done;  // no BVH entries left (or an any-hit query found one)
   bool found;     // any-hit query hit something
-  int spill_code[kSpill];
-  float spill_t[kSpill];
 
   ZT_D void start(const SceneView<R>& sc, const LaneStack& stk, V3<R> o_, V3<R> d_, R tmin_, R tmax, int self_,
                   int excl_, bool anyhit_) {
@@ -315,7 +321,7 @@ struct Walker {
       bvh_done = true;
     }
   }
-  ZT_D void push(const LaneStack& stk, int c, R t, int* overflow) {
+  ZT_D void push(const LaneStack& stk, SpillStack& sv, int c, R t, int* overflow) {
     const int half = stk.cap >> 1;
     if (sp == stk.cap) {
       if (nsp + half > kSpill) {
@@ -323,8 +329,8 @@ struct Walker {
         return;
       }
       for (int i = 0; i < half; ++i) {
-        spill_code[nsp + i] = stk.code[i * stk.stride];
-        spill_t[nsp + i] = stk.tval[i * stk.stride];
+        sv.code[nsp + i] = stk.code[i * stk.stride];
+        sv.t[nsp + i] = stk.tval[i * stk.stride];
         stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
         stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
       }
@@ -336,7 +342,7 @@ struct Walker {
     ++sp;
   }
   // One pop: an interior node is traversed, a leaf is parked in `leaf`.
-  ZT_D void inner(const SceneView<R>& sc, const LaneStack& stk, int* overflow) {
+  ZT_D void inner(const SceneView<R>& sc, const LaneStack& stk, SpillStack& sv, int* overflow) {
     if (sp == 0) {
       if (nsp == 0) {
         bvh_done = true;
@@ -345,8 +351,8 @@ struct Walker {
       const int half = stk.cap >> 1;
       nsp -= half;
       for (int i = 0; i < half; ++i) {
-        stk.code[i * stk.stride] = spill_code[nsp + i];
-        stk.tval[i * stk.stride] = spill_t[nsp + i];
+        stk.code[i * stk.stride] = sv.code[nsp + i];
+        stk.tval[i * stk.stride] = sv.t[nsp + i];
       }
       sp = half;
     }
@@ -366,12 +372,12 @@ struct Walker {
         slab<R>(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, o, inv, nx, ny, nz, tmin, closest, &tr) && n.d.y != kNoChild;
     if (hl && hr) {
       const bool near_left = tl <= tr;
-      push(stk, near_left ? n.d.y : n.d.x, near_left ? tr : tl, overflow);
-      push(stk, near_left ? n.d.x : n.d.y, near_left ? tl : tr, overflow);
+      push(stk, sv, near_left ? n.d.y : n.d.x, near_left ? tr : tl, overflow);
+      push(stk, sv, near_left ? n.d.x : n.d.y, near_left ? tl : tr, overflow);
     } else if (hl) {
-      push(stk, n.d.x, tl, overflow);
+      push(stk, sv, n.d.x, tl, overflow);
     } else if (hr) {
-      push(stk, n.d.y, tr, overflow);
+      push(stk, sv, n.d.y, tr, overflow);
     }
   }
   ZT_D void leafStep(const SceneView<R>& sc) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <tuple>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -97,6 +99,20 @@ struct WfBuffers {
   DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, counters;
 };
 
+// Autotuner record: the fastest configuration of one method for one class of
+// render (image size and primitive count class, i.e. bit length of the count).
+struct TuneKey {
+  uint32_t method, precision, width, height, prim_class, brute;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(method, precision, width, height, prim_class, brute) <
+           std::tie(o.method, o.precision, o.width, o.height, o.prim_class, o.brute);
+  }
+};
+struct TuneEntry {
+  zt_gpu_tuning tu;
+  double ms;
+};
+
 }  // namespace ztg
 
 struct zt_ctx {
@@ -108,6 +124,7 @@ struct zt_ctx {
   zt_gpu_stats stats{};
   ztg::DevBuf out, errflag, scratch;
   ztg::WfBuffers wf;
+  std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };
 
 namespace ztg {

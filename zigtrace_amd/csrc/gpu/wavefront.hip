@@ -161,6 +161,7 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
   uint32_t* fetch = &w.counters[shadow ? 9 : 8];
   const uint32_t lane = __lane_id();
   Walker<R> wk;
+  SpillStack sv;
   const SceneView<R>* v = nullptr;
   uint32_t slot = 0;
   bool has = false, exhausted = false;
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
     if (__ballot(has) == 0) break;
     for (;;) {
       if (!(has && !wk.bvh_done && wk.leaf == kNoChild)) break;
-      wk.inner(*v, stk, &overflow);
+      wk.inner(*v, stk, sv, &overflow);
     }
     if (has && wk.leaf != kNoChild) wk.leafStep(*v);
     if (has && wk.bvh_done) {

@@ -135,6 +135,16 @@ class Context:
         self._check(N.lib().zt_gpu_render(self._h, scene_id, N.ptr(opts.record()), N.ptr(t.record()),
                                           C.c_void_p(device_ptr), 1))
 
+    def autotune(self, scene_id: int, opts: Options, method: int = N.METHOD_AUTO,
+                 precision: int = N.PRECISION_FP32, probe_spp: int = 0) -> tuple[Tuning, float]:
+        """Times the candidate launch configurations on a reduced-spp probe of this frame and
+        records the fastest in the context (used by later AUTO renders). Returns (tuning, ms)."""
+        rec = np.zeros(1, dtype=N.TUNING)
+        ms = C.c_double()
+        self._check(N.lib().zt_gpu_autotune(self._h, scene_id, N.ptr(opts.record()), method, precision, probe_spp,
+                                            N.ptr(rec), C.byref(ms)))
+        return Tuning(**{k: int(rec[k][0]) for k in N.TUNING.names}), ms.value
+
     def render_batch(self, scene_ids, options, tuning: Tuning | None = None, outs=None):
         """Renders several scenes in one wavefront pass; returns host images."""
         t = tuning or Tuning(method=N.METHOD_WAVEFRONT)
