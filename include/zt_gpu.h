@@ -72,6 +72,9 @@ typedef struct zt_gpu_tuning {
                              2/3/4 = compiled for >= 4/5/6 waves per SIMD */
   uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64
                              (0 = megakernel fp32: 8, fp64: 16; wavefront: 16) */
+  uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
+                             nodes (the fp64 path always walks the two-child nodes) */
+  uint32_t _reserved;
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */
@@ -143,8 +146,10 @@ int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opt
 int zt_gpu_trace_rays(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, uint64_t* rng_states,
                       uint64_t max_depth, uint32_t light_sampling, uint32_t precision, double* out_rgb);
 
-/* Scene.intersect (scene.zig:108-116) for n rays: mode 0 = BVH + planes,
- * mode 1 = brute force over every primitive (geometry.zig:129-139). */
+/* Scene.intersect (scene.zig:108-116) for n rays: mode 0 = BVH + planes as
+ * the renderer traverses it (fp32: the four-wide BVH when the scene has one),
+ * mode 1 = brute force over every primitive (geometry.zig:129-139), mode 2 =
+ * the two-child BVH + planes. */
 int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, double t_min, double t_max,
                      int mode, uint32_t precision, zt_hit* out);
 

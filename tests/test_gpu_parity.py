@@ -46,14 +46,15 @@ def test_fp32_uniform_is_truncated_f64_uniform(gpu_ctx):
         assert np.all((bits[i] >= 0) & (bits[i] < 1))
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 7, 50, 400])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 50, 400, 5000])
 @pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
-def test_bvh_traversal_equals_brute_force(gpu_ctx, n, precision):
+@pytest.mark.parametrize("mode", [0, 2])  # the renderer's traversal (four-wide on fp32); the two-child one
+def test_bvh_traversal_equals_brute_force(gpu_ctx, n, precision, mode):
     rng = np.random.default_rng(2026 + n)
     sc = random_scene(rng, n)
     sid = gpu_ctx.upload(sc)
     rays = aimed_rays(rng, sc.view_arrays()["shapes"], 3000)
-    bvh = gpu_ctx.intersect(sid, rays, 1e-6, 1e9, mode=0, precision=precision)
+    bvh = gpu_ctx.intersect(sid, rays, 1e-6, 1e9, mode=mode, precision=precision)
     brute = gpu_ctx.intersect(sid, rays, 1e-6, 1e9, mode=1, precision=precision)
     gpu_ctx.release(sid)
     assert np.array_equal(bvh["hit"], brute["hit"])

@@ -66,11 +66,29 @@ def test_kernel_variants_and_stack_sizes_are_bit_identical(precision):
         assert np.array_equal(base, img), cap
 
 
+def test_two_child_and_four_wide_traversal_agree():
+    """fp32 renders walk the four-wide BVH by default; the two-child walk finds the
+    same closest primitives (exact t ties aside), so the images agree; fp64 always
+    walks the two-child nodes, so bvh_width cannot change it."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=4, width=96, height=54)
+    s, o = wl.scene, wl.options
+    four = zt.render(s, o, zt.Tuning(method=MEGA))
+    two = zt.render(s, o, zt.Tuning(method=MEGA, bvh_width=2))
+    differ = np.any(four != two, axis=2).mean()
+    assert differ <= 1e-3, differ
+    assert np.array_equal(four, zt.render(s, o, zt.Tuning(method=MEGA, bvh_width=4)))
+    assert np.array_equal(two, zt.render(s, o, zt.Tuning(method=WAVE, bvh_width=2, queue_paths=1000)))
+    f64 = [zt.render(s, o, zt.Tuning(method=MEGA, precision=N.PRECISION_FP64, bvh_width=w)) for w in (2, 4)]
+    assert np.array_equal(f64[0], f64[1])
+
+
 def test_rejects_bad_tuning():
     s = zt.parse(scene_text("cornell"))
     o = zt.Options(8, 8, 1, 4)
     for tu in (zt.Tuning(method=MEGA, stack_entries=7), zt.Tuning(method=MEGA, stack_entries=66),
-               zt.Tuning(method=MEGA, variant=9), zt.Tuning(method=WAVE, stack_entries=5)):
+               zt.Tuning(method=MEGA, variant=9), zt.Tuning(method=WAVE, stack_entries=5),
+               zt.Tuning(method=MEGA, bvh_width=3)):
         with pytest.raises(zt.ZTError) as e:
             zt.render(s, o, tu)
         assert e.value.name == "InvalidArgument"

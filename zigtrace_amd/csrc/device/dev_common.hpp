@@ -149,6 +149,18 @@ struct __align__(16) DNode {
   int4 d;
 };
 static constexpr int kNoChild = INT32_MIN;
+
+// Four-wide node of the fp32 product path: one 128-byte line holding four
+// child boxes (the same conservatively rounded fp32 boxes as the two-child
+// nodes, laid out per coordinate so a node is eight 16-byte loads) and their
+// codes (as in DNode; code >= 0 indexes the four-wide array). Empty slots
+// have code kNoChild and an inverted box.
+struct __align__(128) DNode4 {
+  float4 lox, loy, loz, hix, hiy, hiz;
+  int4 child;
+  int4 _pad;
+};
+static_assert(sizeof(DNode4) == 128, "one four-wide node per 128-byte line");
 static constexpr uint32_t kMaxLeafCount = 15;
 static constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
 ZT_HD int leafCode(uint32_t first, uint32_t count) { return -1 - (int)((first << 4) | count); }
@@ -210,6 +222,7 @@ struct CamD {
 template <typename R>
 struct SceneView {
   const DNode* nodes;
+  const DNode4* nodes4;  // fp32 only: four-wide BVH (null: two-child traversal)
   const void* prims;  // PrimF for float, PrimD for double
   const Plane<R>* planes;
   const Mat<R>* mats;

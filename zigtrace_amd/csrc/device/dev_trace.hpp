@@ -135,8 +135,8 @@ ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
 static constexpr int kLds = 16;  // default LDS entries per lane
 static constexpr int kSpill = 80;
 // Deepest device tree the stack can traverse for the smallest LDS capacity
-// used (8 entries), checked at upload: a BVH needs at most depth + 1
-// entries with nearest-first pushes.
+// used (8 entries), checked at upload: a two-child BVH needs at most
+// depth + 1 entries with nearest-first pushes, a four-wide one 3 * depth + 1.
 static constexpr int kMaxTreeDepth = 8 + kSpill - 1;
 
 struct LaneStack {
@@ -156,6 +156,52 @@ template <typename R>
 ZT_D float entryT(R t) {
   if constexpr (K<R>::exact) return __double2float_rd(t);  // rounding down keeps pruning conservative
   else return t;
+}
+
+// Tests the primitives of leaf `code` in slot order (scene.zig:100-104 via
+// bvh.zig:108-117), shrinking *closest. Returns true when an any-hit query
+// found its hit.
+template <typename R>
+ZT_D bool leafHits(const SceneView<R>& sc, int code, V3<R> o, V3<R> d, R tmin, R* closest, int* best, int self,
+                   int excl, bool anyhit) {
+  constexpr bool kIdentity = !K<R>::exact;
+  const int v = -1 - code;
+  const int first = v >> 4, count = v & 15;
+  for (int k = first; k < first + count; ++k) {
+    if (kIdentity && k == excl) continue;
+    const Prim<R> p = loadPrim<R>(sc, k);
+    R t;
+    bool h;
+    if (metaKind(p.meta) == kKindTriangle) {
+      h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, *closest, &t);
+    } else {
+      h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, *closest, &t)
+                                   : sphereHit<R>(p, o, d, tmin, *closest, &t);
+    }
+    if (h) {
+      *closest = t;
+      *best = k;
+      if (anyhit) return true;
+    }
+  }
+  return false;
+}
+
+// Planes after the BVH (scene.zig:113-114).
+template <typename R>
+ZT_D void planeHits(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R* closest, int* best, int self, int excl,
+                    bool anyhit) {
+  constexpr bool kIdentity = !K<R>::exact;
+  for (int j = 0; j < sc.num_planes; ++j) {
+    const int id = sc.num_prims + j;
+    if (kIdentity && (id == self || id == excl)) continue;
+    R t;
+    if (planeHit<R>(sc.planes[j], o, d, tmin, *closest, &t)) {
+      *closest = t;
+      *best = id;
+      if (anyhit) return;
+    }
+  }
 }
 
 // Nearest-hit (anyhit == false) or any-hit query over BVH + planes.
@@ -228,109 +274,43 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
         } else if (hr) {
           push(n.d.y, tr);
         }
-      } else {
-        const int v = -1 - code;
-        const int first = v >> 4, count = v & 15;
-        for (int k = first; k < first + count; ++k) {
-          if (kIdentity && k == excl) continue;
-          const Prim<R> p = loadPrim<R>(sc, k);
-          R t;
-          bool h;
-          if (metaKind(p.meta) == kKindTriangle) {
-            h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
-          } else {
-            h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
-                                         : sphereHit<R>(p, o, d, tmin, closest, &t);
-          }
-          if (h) {
-            closest = t;
-            best = k;
-            if (anyhit) {
-              done = true;
-              break;
-            }
-          }
-        }
-        if (done) break;
+      } else if (leafHits<R>(sc, code, o, d, tmin, &closest, &best, self, excl, anyhit)) {
+        done = true;
+        break;
       }
     }
   }
-  if (!done) {
-    for (int j = 0; j < sc.num_planes; ++j) {
-      const int id = sc.num_prims + j;
-      if (kIdentity && (id == self || id == excl)) continue;
-      R t;
-      if (planeHit<R>(sc.planes[j], o, d, tmin, closest, &t)) {
-        closest = t;
-        best = id;
-        if (anyhit) break;
-      }
-    }
-  }
+  if (!done) planeHits<R>(sc, o, d, tmin, &closest, &best, self, excl, anyhit);
   return HitRec<R>{closest, best};
 }
 
-// Resumable form of `traverse` for kernels whose lanes take a new ray as
-// soon as theirs is finished (dynamic ray fetch). It performs the same
-// sequence of stack operations, box tests and primitive tests as `traverse`,
-// split into an interior phase (`inner`, called until a leaf is popped or the
-// stack is empty) and a leaf phase (`leafStep`), so the hit is identical.
-// Overflow of a lane's LDS stack into private memory. Kept apart from the
-// Walker so the Walker's scalar state can live in registers (an aggregate
-// with a dynamically indexed array member is kept whole in private memory).
-struct SpillStack {
-  int code[kSpill];
-  float t[kSpill];
-};
-
-template <typename R>
-struct Walker {
-  V3<R> o, d, inv;
-  R tmin, closest;
-  int best, self, excl;
-  int sp, nsp;
-  int leaf;  // popped leaf code waiting for leafStep, or kNoChild
-  bool nx, ny, nz, anyhit;
-  bool bvh_done;  // no BVH entries left (or an any-hit query found one)
-  bool found;     // any-hit query hit something
-
-  ZT_D void start(const SceneView<R>& sc, const LaneStack& stk, V3<R> o_, V3<R> d_, R tmin_, R tmax, int self_,
-                  int excl_, bool anyhit_) {
-    o = o_;
-    d = d_;
-    tmin = tmin_;
-    closest = tmax;
-    best = -1;
-    self = self_;
-    excl = excl_;
-    anyhit = anyhit_;
-    found = false;
-    leaf = kNoChild;
-    nsp = 0;
-    inv = mk(R(1) / d.x, R(1) / d.y, R(1) / d.z);
-    nx = inv.x < 0;
-    ny = inv.y < 0;
-    nz = inv.z < 0;
-    if (sc.num_nodes > 0) {
-      stk.code[0] = 0;
-      stk.tval[0] = entryT<R>(tmin);
-      sp = 1;
-      bvh_done = false;
-    } else {
-      sp = 0;
-      bvh_done = true;
-    }
-  }
-  ZT_D void push(const LaneStack& stk, SpillStack& sv, int c, R t, int* overflow) {
-    const int half = stk.cap >> 1;
+// Four-wide traversal of the fp32 product path over the same fp32 boxes and
+// primitive tests as `traverse`: a node's hit children are ordered by entry
+// distance with a sorting network, the farther ones pushed and the nearest
+// visited next without a stack round trip; popped entries beyond the closest
+// hit are pruned. The closest hit is the primitive the two-child traversal
+// finds, except that two primitives at exactly the same t may be resolved the
+// other way (the visiting order differs).
+ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
+                             float tmax, int self, int excl, bool anyhit, int* overflow) {
+  float closest = tmax;
+  int best = -1;
+  bool done = false;
+  const V3<float> inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+  int spill_code[kSpill];
+  float spill_t[kSpill];
+  int sp = 0, nsp = 0;
+  const int half = stk.cap >> 1;
+  auto push = [&](int c, float t) {
     if (sp == stk.cap) {
       if (nsp + half > kSpill) {
         *overflow = 1;
         return;
       }
       for (int i = 0; i < half; ++i) {
-        sv.code[nsp + i] = stk.code[i * stk.stride];
-        sv.t[nsp + i] = stk.tval[i * stk.stride];
+        spill_code[nsp + i] = stk.code[i * stk.stride];
+        spill_t[nsp + i] = stk.tval[i * stk.stride];
         stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
         stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
       }
@@ -338,93 +318,69 @@ struct Walker {
       sp = half;
     }
     stk.code[sp * stk.stride] = c;
-    stk.tval[sp * stk.stride] = entryT<R>(t);
+    stk.tval[sp * stk.stride] = t;
     ++sp;
-  }
-  // One pop: an interior node is traversed, a leaf is parked in `leaf`.
-  ZT_D void inner(const SceneView<R>& sc, const LaneStack& stk, SpillStack& sv, int* overflow) {
-    if (sp == 0) {
-      if (nsp == 0) {
-        bvh_done = true;
-        return;
-      }
-      const int half = stk.cap >> 1;
-      nsp -= half;
-      for (int i = 0; i < half; ++i) {
-        stk.code[i * stk.stride] = sv.code[nsp + i];
-        stk.tval[i * stk.stride] = sv.t[nsp + i];
-      }
-      sp = half;
-    }
-    --sp;
-    const int code = stk.code[sp * stk.stride];
-    const float et = stk.tval[sp * stk.stride];
-    if (R(et) > closest) return;
-    if (code < 0) {
-      leaf = code;
-      return;
-    }
-    const DNode n = sc.nodes[code];
-    R tl, tr;
-    const bool hl =
-        slab<R>(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, o, inv, nx, ny, nz, tmin, closest, &tl) && n.d.x != kNoChild;
-    const bool hr =
-        slab<R>(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, o, inv, nx, ny, nz, tmin, closest, &tr) && n.d.y != kNoChild;
-    if (hl && hr) {
-      const bool near_left = tl <= tr;
-      push(stk, sv, near_left ? n.d.y : n.d.x, near_left ? tr : tl, overflow);
-      push(stk, sv, near_left ? n.d.x : n.d.y, near_left ? tl : tr, overflow);
-    } else if (hl) {
-      push(stk, sv, n.d.x, tl, overflow);
-    } else if (hr) {
-      push(stk, sv, n.d.y, tr, overflow);
-    }
-  }
-  ZT_D void leafStep(const SceneView<R>& sc) {
-    constexpr bool kIdentity = !K<R>::exact;
-    const int v = -1 - leaf;
-    leaf = kNoChild;
-    const int first = v >> 4, count = v & 15;
-    for (int k = first; k < first + count; ++k) {
-      if (kIdentity && k == excl) continue;
-      const Prim<R> p = loadPrim<R>(sc, k);
-      R t;
-      bool h;
-      if (metaKind(p.meta) == kKindTriangle) {
-        h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
-      } else {
-        h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
-                                     : sphereHit<R>(p, o, d, tmin, closest, &t);
-      }
-      if (h) {
-        closest = t;
-        best = k;
-        if (anyhit) {
-          found = true;
-          bvh_done = true;
-          return;
+  };
+  auto cswap = [](float& ka, int& ca, float& kb, int& cb) {
+    const bool sw = ka > kb;
+    const float k = sw ? kb : ka;
+    const int c = sw ? cb : ca;
+    kb = sw ? ka : kb;
+    cb = sw ? ca : cb;
+    ka = k;
+    ca = c;
+  };
+  int cur = 0;  // root
+  for (;;) {
+    if (cur == kNoChild) {
+      if (sp == 0) {
+        if (nsp == 0) break;
+        nsp -= half;
+        for (int i = 0; i < half; ++i) {
+          stk.code[i * stk.stride] = spill_code[nsp + i];
+          stk.tval[i * stk.stride] = spill_t[nsp + i];
         }
+        sp = half;
       }
+      --sp;
+      const float et = stk.tval[sp * stk.stride];
+      if (et > closest) continue;
+      cur = stk.code[sp * stk.stride];
     }
-  }
-  // Planes after the BVH (scene.zig:113-114), then the result.
-  ZT_D HitRec<R> finish(const SceneView<R>& sc) {
-    constexpr bool kIdentity = !K<R>::exact;
-    if (!found) {
-      for (int j = 0; j < sc.num_planes; ++j) {
-        const int id = sc.num_prims + j;
-        if (kIdentity && (id == self || id == excl)) continue;
-        R t;
-        if (planeHit<R>(sc.planes[j], o, d, tmin, closest, &t)) {
-          closest = t;
-          best = id;
-          if (anyhit) break;
-        }
-      }
+    if (cur >= 0) {
+      const DNode4* n = sc.nodes4 + cur;
+      const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
+      int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
+      float k0, k1, k2, k3;
+      const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
+      const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
+      const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
+      const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
+      // Misses (and empty slots, whose boxes are inverted) sort last.
+      if (!(h0 && c0 != kNoChild)) k0 = INFINITY, c0 = kNoChild;
+      if (!(h1 && c1 != kNoChild)) k1 = INFINITY, c1 = kNoChild;
+      if (!(h2 && c2 != kNoChild)) k2 = INFINITY, c2 = kNoChild;
+      if (!(h3 && c3 != kNoChild)) k3 = INFINITY, c3 = kNoChild;
+      cswap(k0, c0, k1, c1);
+      cswap(k2, c2, k3, c3);
+      cswap(k0, c0, k2, c2);
+      cswap(k1, c1, k3, c3);
+      cswap(k1, c1, k2, c2);
+      if (c3 != kNoChild) push(c3, k3);
+      if (c2 != kNoChild) push(c2, k2);
+      if (c1 != kNoChild) push(c1, k1);
+      cur = c0;  // kNoChild when nothing was hit: pop next
+      continue;
     }
-    return HitRec<R>{closest, best};
+    if (leafHits<float>(sc, cur, o, d, tmin, &closest, &best, self, excl, anyhit)) {
+      done = true;
+      break;
+    }
+    cur = kNoChild;
   }
-};
+  if (!done) planeHits<float>(sc, o, d, tmin, &closest, &best, self, excl, anyhit);
+  return HitRec<float>{closest, best};
+}
 
 // Brute force over every primitive in slot order, then planes
 // (geometry.zig:129-139 and scene.zig:109-117): the reference the BVH
@@ -464,6 +420,19 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
     }
   }
   return HitRec<R>{closest, best};
+}
+
+// One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force
+// when the scene has no BVH, the four-wide BVH on the fp32 path when the view
+// carries one, else the two-child traversal.
+template <typename R>
+ZT_D HitRec<R> query(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V3<R> d, R tmin, R tmax, int self,
+                     int excl, bool anyhit, int* overflow) {
+  if (sc.num_nodes == 0 && sc.num_prims > 0) return bruteForce<R>(sc, o, d, tmin, tmax, self, excl, anyhit);
+  if constexpr (!K<R>::exact) {
+    if (sc.nodes4) return traverse4(sc, stk, o, d, tmin, tmax, self, excl, anyhit, overflow);
+  }
+  return traverse<R>(sc, stk, o, d, tmin, tmax, self, excl, anyhit, overflow);
 }
 
 // Surface data of a hit (geometry.zig:20-29): point, ray-facing normal,

@@ -185,12 +185,8 @@ __global__ __launch_bounds__(256) void adjointKernel(SceneView<R> sc, Params pr,
       int phase = beginPath<R>(sc, pr, x, y, rng, st);
       while (phase != kFinish) {
         const bool sh = phase == kShadow;
-        const HitRec<R> h =
-            (sc.num_nodes == 0 && sc.num_prims > 0)
-                ? bruteForce<R>(sc, st.ro, sh ? st.sh_d : st.rd, R(kRayEpsA), sh ? st.sh_tmax : R(INFINITY), st.self,
-                                sh ? st.sh_excl : -1, sh)
-                : traverse<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, R(kRayEpsA), sh ? st.sh_tmax : R(INFINITY),
-                              st.self, sh ? st.sh_excl : -1, sh, &overflow);
+        const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, R(kRayEpsA), sh ? st.sh_tmax : R(INFINITY),
+                                     st.self, sh ? st.sh_excl : -1, sh, &overflow);
         phase = sh ? afterShadow<R>(h, st, &obs) : shade<R>(sc, pr, h, rng, st, &obs);
       }
       if (isFinite(st.radiance)) obs.flush(sc);  // the forward drops non-finite samples (render.zig:141-142)
@@ -257,9 +253,11 @@ void setParams(zt_ctx* ctx, DevScene& s, const float* params) {
 }
 
 template <typename R, int K>
-void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const float* ddimg, R* dgrad) {
+void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, const float* ddimg,
+                   R* dgrad) {
   SceneView<R> view = viewOf<R>(s);
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
+  applyBvhWidth(view, tu);  // the replay must walk the tree the forward render walked
   const Params pr = makeParams(s, o);
   const uint32_t tw = 8, threads = 64;
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;
@@ -354,13 +352,13 @@ int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opt
     else hipLaunchKernelGGL(zeroFill<float>, dim3(1), dim3(256), 0, ctx->stream, grad_buf.as<float>(), np);
     const uint64_t d = opts->max_depth;
     if (f64) {
-      if (d <= 8) launchAdjoint<double, 8>(ctx, s, *opts, ddimg, grad_buf.as<double>());
-      else if (d <= 16) launchAdjoint<double, 16>(ctx, s, *opts, ddimg, grad_buf.as<double>());
-      else launchAdjoint<double, 32>(ctx, s, *opts, ddimg, grad_buf.as<double>());
+      if (d <= 8) launchAdjoint<double, 8>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
+      else if (d <= 16) launchAdjoint<double, 16>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
+      else launchAdjoint<double, 32>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
     } else {
-      if (d <= 8) launchAdjoint<float, 8>(ctx, s, *opts, ddimg, grad_buf.as<float>());
-      else if (d <= 16) launchAdjoint<float, 16>(ctx, s, *opts, ddimg, grad_buf.as<float>());
-      else launchAdjoint<float, 32>(ctx, s, *opts, ddimg, grad_buf.as<float>());
+      if (d <= 8) launchAdjoint<float, 8>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
+      else if (d <= 16) launchAdjoint<float, 16>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
+      else launchAdjoint<float, 32>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
     }
     ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<double> gd(np);

@@ -14,6 +14,12 @@
 #include "../device/dev_common.hpp"
 #include "zt_gpu.h"
 
+// The ABI records keep the sizes the bindings are written against.
+static_assert(sizeof(zt_gpu_tuning) == 40, "zt_gpu_tuning layout");
+static_assert(sizeof(zt_gpu_stats) == 48, "zt_gpu_stats layout");
+static_assert(sizeof(zt_render_options) == 64, "zt_render_options layout");
+static_assert(sizeof(zt_ray) == 48 && sizeof(zt_hit) == 72, "zt_ray / zt_hit layout");
+
 namespace ztg {
 
 // Thrown inside the library, converted to a status code at the ABI edge.
@@ -77,9 +83,9 @@ struct LightCandidate : ztd::Light<double> {
 // One uploaded scene: the device layout in both precisions plus what the
 // host needs to rebuild per-render constants.
 struct DevScene {
-  DevBuf nodes, primsF, primsD, planesF, planesD, matsF, matsD, lightsF, lightsD, cdf, prim_light;
-  int num_nodes = 0, num_prims = 0, num_planes = 0, num_lights = 0, num_mats = 0;
-  int tree_depth = 0;
+  DevBuf nodes, nodes4, primsF, primsD, planesF, planesD, matsF, matsD, lightsF, lightsD, cdf, prim_light;
+  int num_nodes = 0, num_nodes4 = 0, num_prims = 0, num_planes = 0, num_lights = 0, num_mats = 0;
+  int tree_depth = 0, tree_depth4 = 0;
   bool brute_force = false;  // no BVH handed over (or one too deep for the stack)
   double light_total = 0;
   zt_camera_config camera{};

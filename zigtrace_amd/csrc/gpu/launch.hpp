@@ -18,6 +18,15 @@ void rebuildLights(DevScene& s, hipStream_t stream);
 TuneKey tuneKey(const DevScene& s, const zt_render_options& o, uint32_t method, uint32_t precision);
 zt_gpu_tuning resolveTuning(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning* t);
 
+// Applies zt_gpu_tuning.bvh_width to a view (0 or 4: keep the four-wide
+// nodes if the scene has them; 2: two-child traversal).
+template <typename R>
+void applyBvhWidth(ztd::SceneView<R>& v, const zt_gpu_tuning& tu) {
+  if (tu.bvh_width != 0 && tu.bvh_width != 2 && tu.bvh_width != 4)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "bvh_width must be 0, 2 or 4");
+  if (tu.bvh_width == 2) v.nodes4 = nullptr;
+}
+
 template <typename R>
 void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout);
 template <typename R>

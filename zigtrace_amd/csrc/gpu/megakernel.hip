@@ -22,10 +22,9 @@ namespace ztg {
 static constexpr double kRayEpsilon = 1e-4;  // render.zig:26
 
 template <typename R>
-ZT_D HitRec<R> query(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V3<R> d, R tmax, int self, int excl,
-                     bool anyhit, int* overflow) {
-  if (sc.num_nodes == 0 && sc.num_prims > 0) return bruteForce<R>(sc, o, d, R(kRayEpsilon), tmax, self, excl, anyhit);
-  return traverse<R>(sc, stk, o, d, R(kRayEpsilon), tmax, self, excl, anyhit, overflow);
+ZT_D HitRec<R> rayQuery(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V3<R> d, R tmax, int self, int excl,
+                        bool anyhit, int* overflow) {
+  return query<R>(sc, stk, o, d, R(kRayEpsilon), tmax, self, excl, anyhit, overflow);
 }
 
 // Maps block b to a pixel tile so that the blocks the hardware places on one
@@ -60,7 +59,7 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
     }
     if (phase == kExtend || phase == kShadow) {
       const bool sh = phase == kShadow;
-      const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
+      const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
                                    sh ? st.sh_excl : -1, sh, &overflow);
       phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
     }
@@ -112,7 +111,7 @@ __global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, c
   int overflow = 0;
   while (phase != kFinish) {
     const bool sh = phase == kShadow;
-    const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
+    const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
                                  sh ? st.sh_excl : -1, sh, &overflow);
     phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
   }
@@ -139,9 +138,9 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
   const V3<R> o = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
   const V3<R> d = mk((R)rays[i].dir[0], (R)rays[i].dir[1], (R)rays[i].dir[2]);
   int overflow = 0;
-  const HitRec<R> h = (mode == 1 || (sc.num_nodes == 0 && sc.num_prims > 0))
-                          ? bruteForce<R>(sc, o, d, tmin, tmax, -1, -1, false)
-                          : traverse<R>(sc, stk, o, d, tmin, tmax, -1, -1, false, &overflow);
+  if (mode == 2) sc.nodes4 = nullptr;  // the two-child traversal
+  const HitRec<R> h = mode == 1 ? bruteForce<R>(sc, o, d, tmin, tmax, -1, -1, false)
+                                : query<R>(sc, stk, o, d, tmin, tmax, -1, -1, false, &overflow);
   zt_hit r{};
   r.hit = h.prim >= 0 ? 1 : 0;
   if (h.prim >= 0) {
@@ -201,6 +200,7 @@ template <typename R>
 void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout) {
   SceneView<R> view = viewOf<R>(s);
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);  // render.zig:157,162
+  applyBvhWidth(view, tu);
   Params pr = makeParams(s, o);
   // Defaults measured on the 1080p mesh field (1.3M triangles): 16x16 tiles;
   // fp32 compiled for 6 waves per SIMD with 8 LDS stack entries per lane (LDS
@@ -342,7 +342,7 @@ int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_
   return guarded(ctx, [&] {
     if (n == 0) return;
     if (!rays || !out) throw Error(ZT_ERR_INVALID_ARGUMENT, "null buffer");
-    if (precision > ZT_PRECISION_FP64 || mode < 0 || mode > 1) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad enum");
+    if (precision > ZT_PRECISION_FP64 || mode < 0 || mode > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad enum");
     DevScene& s = sceneOf(ctx, scene_id);
     setDevice(ctx);
     DevBuf drays, dout;

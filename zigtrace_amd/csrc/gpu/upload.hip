@@ -63,6 +63,7 @@ SceneView<R> viewOf(const DevScene& s) {
   SceneView<R> v{};
   v.nodes = s.brute_force ? nullptr : s.nodes.as<DNode>();
   v.num_nodes = s.brute_force ? 0 : s.num_nodes;
+  v.nodes4 = (std::is_same<R, float>::value && !s.brute_force && s.num_nodes4) ? s.nodes4.as<DNode4>() : nullptr;
   if constexpr (std::is_same<R, float>::value) {
     v.prims = s.primsF.p;
     v.planes = s.planesF.as<Plane<float>>();
@@ -185,6 +186,80 @@ struct TreeBuilder {
     } else {
       node(0, 0);
     }
+  }
+};
+
+// Collapses the two-child device tree into four-wide nodes (DNode4): each
+// four-wide node takes a two-child node's children and, while it has fewer
+// than four, replaces its interior child of largest surface area by that
+// child's two children. Nodes are emitted in depth-first preorder; boxes are
+// copied bit for bit from the two-child nodes.
+struct Collapse4 {
+  const std::vector<DNode>& in;
+  std::vector<DNode4> out;
+  int max_depth = 0;
+
+  struct Child {
+    int code;
+    Box6 box;
+  };
+  static Box6 leftBox(const DNode& n) {
+    return Box6{{n.a.x, n.a.y, n.a.z}, {n.a.w, n.b.x, n.b.y}};
+  }
+  static Box6 rightBox(const DNode& n) {
+    return Box6{{n.b.z, n.b.w, n.c.x}, {n.c.y, n.c.z, n.c.w}};
+  }
+  static double area(const Box6& b) {
+    const double dx = (double)b.hi[0] - b.lo[0], dy = (double)b.hi[1] - b.lo[1], dz = (double)b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  }
+  int node(int i, int depth) {
+    max_depth = std::max(max_depth, depth);
+    std::vector<Child> ch;
+    const DNode& n = in[i];
+    if (n.d.x != kNoChild) ch.push_back({n.d.x, leftBox(n)});
+    if (n.d.y != kNoChild) ch.push_back({n.d.y, rightBox(n)});
+    while (ch.size() < 4) {
+      int pick = -1;
+      double best = -1;
+      for (size_t k = 0; k < ch.size(); ++k)
+        if (ch[k].code >= 0 && area(ch[k].box) > best) {
+          best = area(ch[k].box);
+          pick = (int)k;
+        }
+      if (pick < 0) break;
+      const DNode& c = in[ch[pick].code];
+      std::vector<Child> repl;
+      if (c.d.x != kNoChild) repl.push_back({c.d.x, leftBox(c)});
+      if (c.d.y != kNoChild) repl.push_back({c.d.y, rightBox(c)});
+      ch.erase(ch.begin() + pick);
+      ch.insert(ch.begin() + pick, repl.begin(), repl.end());
+    }
+    const int idx = (int)out.size();
+    out.push_back(DNode4{});
+    float lo[3][4], hi[3][4];
+    int code[4];
+    for (int k = 0; k < 4; ++k) {
+      const Box6 b = k < (int)ch.size() ? ch[k].box : emptyBox();
+      for (int a = 0; a < 3; ++a) {
+        lo[a][k] = b.lo[a];
+        hi[a][k] = b.hi[a];
+      }
+      code[k] = k < (int)ch.size() ? ch[k].code : kNoChild;
+    }
+    for (int k = 0; k < 4; ++k)
+      if (code[k] >= 0) code[k] = node(code[k], depth + 1);
+      else if (code[k] != kNoChild) max_depth = std::max(max_depth, depth + 1);
+    DNode4& o = out[idx];
+    o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    o.child = make_int4(code[0], code[1], code[2], code[3]);
+    o._pad = make_int4(0, 0, 0, 0);
+    return idx;
   }
 };
 
@@ -387,6 +462,14 @@ int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id)
       } else {
         s->num_nodes = (int)tb.out.size();
         s->nodes.upload(tb.out, ctx->stream);
+        // The four-wide traversal pushes up to three entries per level.
+        Collapse4 c4{tb.out, {}, 0};
+        c4.node(0, 0);
+        s->tree_depth4 = c4.max_depth;
+        if (3 * c4.max_depth + 1 <= 8 + kSpill) {
+          s->num_nodes4 = (int)c4.out.size();
+          s->nodes4.upload(c4.out, ctx->stream);
+        }
       }
     } else {
       s->brute_force = n > 0;

@@ -41,8 +41,7 @@ struct WfShared {
   double* hits_d;     // per slot: t in f64 for the fp64 path
   uint32_t* qext[2];  // extension queues (double-buffered)
   uint32_t* qsh;      // shadow queue
-  // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64),
-  // [8] extension fetch counter, [9] shadow fetch counter
+  // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64)
   uint32_t* counters;
   uint32_t num_scenes;
   uint32_t num_slots;
@@ -138,78 +137,35 @@ __global__ void wfInit(WfShared<R> w) {
   }
 }
 
-// Closest-hit (extension queue) or any-hit (shadow queue) queries with
-// dynamic ray fetch: a lane whose ray is finished takes the next queue entry
-// at once (one atomic per wave per refill), so waves stay full while their
-// rays need different numbers of steps. Interior nodes are walked until every
-// busy lane holds a leaf, then leaves are tested (while-while).
+// Closest-hit (extension queue) or any-hit (shadow queue) query of every
+// queued ray, one ray per lane over a grid-stride loop. The first lane also
+// clears the counters that the kernels after it append to.
 template <typename R>
 __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shadow, int cap) {
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, cap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (shadow) {
-      w.counters[8] = 0;  // extension fetch counter of the next iteration
-    } else {
-      w.counters[1 - cur] = 0;  // next extension queue
-      w.counters[2] = 0;        // shadow queue
-      w.counters[9] = 0;        // shadow fetch counter
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !shadow) {
+    w.counters[1 - cur] = 0;  // next extension queue
+    w.counters[2] = 0;        // shadow queue
   }
   const uint32_t n = shadow ? w.counters[2] : w.counters[cur];
   const uint32_t* q = shadow ? w.qsh : w.qext[cur];
-  uint32_t* fetch = &w.counters[shadow ? 9 : 8];
-  const uint32_t lane = __lane_id();
-  Walker<R> wk;
-  SpillStack sv;
-  const SceneView<R>* v = nullptr;
-  uint32_t slot = 0;
-  bool has = false, exhausted = false;
+  const uint32_t stride = gridDim.x * blockDim.x;
   int overflow = 0;
-  for (;;) {
-    const uint64_t need = __ballot(!has && !exhausted);
-    if (need) {
-      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(need));
-      base = __shfl(base, (int)leader);
-      if (!has) {
-        const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (idx < n) {
-          slot = q[idx];
-          const WfSlot<R>& sl = w.slots[slot];
-          v = &w.scenes[sl.scene].view;
-          if (shadow) wk.start(*v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self, sl.p.sh_excl, true);
-          else wk.start(*v, stk, sl.p.ro, sl.p.rd, R(kRayEps), R(INFINITY), sl.p.self, -1, false);
-          has = true;
-          if (v->num_nodes == 0 && v->num_prims > 0) {  // no BVH: brute force (scene.zig:109-112)
-            const HitRec<R> h = bruteForce<R>(*v, wk.o, wk.d, wk.tmin, wk.closest, wk.self, wk.excl, wk.anyhit);
-            wk.closest = h.t;
-            wk.best = h.prim;
-            wk.found = true;  // final: bruteForce already tested the planes
-            wk.bvh_done = true;
-          }
-        }
-      }
-      if (base + (uint32_t)__popcll(need) >= n) exhausted = true;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t slot = q[i];
+    const WfSlot<R>& sl = w.slots[slot];
+    const SceneView<R>& v = w.scenes[sl.scene].view;
+    const HitRec<R> h = shadow ? query<R>(v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
+                                          sl.p.sh_excl, true, &overflow)
+                               : query<R>(v, stk, sl.p.ro, sl.p.rd, R(kRayEps), R(INFINITY), sl.p.self, -1, false,
+                                          &overflow);
+    if constexpr (K<R>::exact) {
+      if (!shadow) w.hits_d[slot] = h.t;
+    } else {
+      if (!shadow) w.hits[slot].x = h.t;
     }
-    if (__ballot(has) == 0) break;
-    for (;;) {
-      if (!(has && !wk.bvh_done && wk.leaf == kNoChild)) break;
-      wk.inner(*v, stk, sv, &overflow);
-    }
-    if (has && wk.leaf != kNoChild) wk.leafStep(*v);
-    if (has && wk.bvh_done) {
-      // found: an any-hit query stopped early, or bruteForce gave the answer.
-      const HitRec<R> h = wk.found ? HitRec<R>{wk.closest, wk.best} : wk.finish(*v);
-      if constexpr (K<R>::exact) {
-        if (!shadow) w.hits_d[slot] = h.t;
-      } else {
-        if (!shadow) w.hits[slot].x = h.t;
-      }
-      w.hits[slot].y = __int_as_float(h.prim);
-      has = false;
-    }
+    w.hits[slot].y = __int_as_float(h.prim);
   }
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
@@ -398,6 +354,7 @@ void launchWavefront(zt_ctx* ctx, const DevScene& s, const zt_render_options& o,
   ws.view = viewOf<R>(s);
   ws.pr = makeParams(s, o);
   ws.view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
+  applyBvhWidth(ws.view, tu);
   ws.out = dout;
   ws.first_pixel = 0;
   runWavefront<R>(ctx, std::vector<WfScene<R>>{ws}, tu);
@@ -449,6 +406,7 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
         const DevScene& s = sceneOf(ctx, scene_ids[i]);
         scenes[i].view = viewOf<R>(s);
         scenes[i].view.cam = cameraInit(s.camera, (double)opts[i].width / (double)opts[i].height);
+        applyBvhWidth(scenes[i].view, tu);
         scenes[i].pr = makeParams(s, opts[i]);
         scenes[i].out = (R*)devOut(i);
         scenes[i].first_pixel = first;
