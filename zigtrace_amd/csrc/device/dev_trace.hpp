@@ -131,7 +131,10 @@ ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1,
 
 // Per-lane traversal stack: the top `cap` entries live in LDS (laid out
 // [entry][lane], 4-byte words, so a wave's accesses are bank-conflict free
-// whatever its lanes' depths), the rest spill in halves to private memory.
+// whatever its lanes' depths), the rest spill in halves to a global buffer
+// the launch provides, kSpill entries per launched thread laid out
+// [entry][thread] (coalesced across a wave). Nothing goes to private scratch,
+// whose size would otherwise cap how many waves run at once.
 static constexpr int kLds = 16;  // default LDS entries per lane
 static constexpr int kSpill = 80;
 // Deepest device tree the stack can traverse for the smallest LDS capacity
@@ -143,13 +146,21 @@ struct LaneStack {
   int* code;    // LDS: code[i * stride]
   float* tval;  // LDS: tval[i * stride]
   int stride;
-  int cap;      // LDS entries per lane (even, >= 8)
+  int cap;           // LDS entries per lane (even, >= 8)
+  int* scode;        // global spill: scode[i * sstride]
+  float* stval;      // global spill: stval[i * sstride]
+  uint32_t sstride;  // threads in the launch
 };
 
 // LDS bytes of a stack of `cap` entries per lane for `threads` lanes.
 ZT_HD size_t stackBytes(int cap, int threads) { return (size_t)2 * cap * threads * sizeof(int); }
-ZT_D LaneStack laneStack(int* lds, int cap) {
-  return LaneStack{lds + threadIdx.x, (float*)(lds + cap * blockDim.x) + threadIdx.x, (int)blockDim.x, cap};
+// Global spill bytes for a launch of `threads` threads.
+ZT_HD size_t spillBytes(uint64_t threads) { return (size_t)2 * kSpill * threads * sizeof(int); }
+ZT_D LaneStack laneStack(int* lds, int cap, int* spill) {
+  const uint32_t threads = gridDim.x * blockDim.x;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  return LaneStack{lds + threadIdx.x, (float*)(lds + cap * blockDim.x) + threadIdx.x, (int)blockDim.x, cap,
+                   spill + gtid, (float*)(spill + (size_t)kSpill * threads) + gtid, threads};
 }
 
 template <typename R>
@@ -218,8 +229,6 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
   if (sc.num_nodes > 0) {
     const V3<R> inv = mk(R(1) / d.x, R(1) / d.y, R(1) / d.z);
     const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    int spill_code[kSpill];
-    float spill_t[kSpill];
     int sp = 0, nsp = 0;
     stk.code[0] = 0;
     stk.tval[0] = entryT<R>(tmin);
@@ -232,8 +241,8 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
           return;
         }
         for (int i = 0; i < half; ++i) {
-          spill_code[nsp + i] = stk.code[i * stk.stride];
-          spill_t[nsp + i] = stk.tval[i * stk.stride];
+          stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
+          stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
           stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
           stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
         }
@@ -249,8 +258,8 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
         if (nsp == 0) break;
         nsp -= half;
         for (int i = 0; i < half; ++i) {
-          stk.code[i * stk.stride] = spill_code[nsp + i];
-          stk.tval[i * stk.stride] = spill_t[nsp + i];
+          stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
+          stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
         }
         sp = half;
       }
@@ -298,8 +307,6 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
   bool done = false;
   const V3<float> inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-  int spill_code[kSpill];
-  float spill_t[kSpill];
   int sp = 0, nsp = 0;
   const int half = stk.cap >> 1;
   auto push = [&](int c, float t) {
@@ -309,8 +316,8 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
         return;
       }
       for (int i = 0; i < half; ++i) {
-        spill_code[nsp + i] = stk.code[i * stk.stride];
-        spill_t[nsp + i] = stk.tval[i * stk.stride];
+        stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
+        stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
         stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
         stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
       }
@@ -337,8 +344,8 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
         if (nsp == 0) break;
         nsp -= half;
         for (int i = 0; i < half; ++i) {
-          stk.code[i * stk.stride] = spill_code[nsp + i];
-          stk.tval[i * stk.stride] = spill_t[nsp + i];
+          stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
+          stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
         }
         sp = half;
       }

@@ -158,10 +158,10 @@ template <typename R, int K>
 __global__ __launch_bounds__(256) void adjointKernel(SceneView<R> sc, Params pr, const float* __restrict__ dimg,
                                                      const int* __restrict__ offsets, int nparams,
                                                      R* __restrict__ grad, uint32_t tile_w, uint32_t tiles_x,
-                                                     uint32_t ntiles, int* err) {
+                                                     uint32_t ntiles, int* spill, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack(lds, kLds);
+  const LaneStack stk = laneStack(lds, kLds, spill);
   R* lg = (R*)(lds + 2 * kLds * blockDim.x);
   for (int i = (int)tid; i < nparams; i += (int)blockDim.x) lg[i] = R(0);
   __syncthreads();
@@ -263,9 +263,10 @@ void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const z
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;
   const uint32_t tiles_y = (pr.height + (threads / tw) - 1) / (threads / tw);
   const size_t lds = stackBytes(kLds, (int)threads) + (size_t)s.num_params * sizeof(R);
+  ctx->spill.ensure(spillBytes((uint64_t)tiles_x * tiles_y * threads));
   hipLaunchKernelGGL((adjointKernel<R, K>), dim3(tiles_x * tiles_y), dim3(threads), lds, ctx->stream, view, pr, ddimg,
                      s.param_offsets.as<int>(), s.num_params, dgrad, tw, tiles_x, tiles_x * tiles_y,
-                     ctx->errflag.as<int>());
+                     ctx->spill.as<int>(), ctx->errflag.as<int>());
   ZT_HIP(hipGetLastError());
 }
 

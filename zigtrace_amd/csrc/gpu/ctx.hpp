@@ -129,6 +129,8 @@ struct zt_ctx {
   std::vector<std::unique_ptr<ztg::DevScene>> scenes;  // index = scene id; null = released
   zt_gpu_stats stats{};
   ztg::DevBuf out, errflag, scratch;
+  ztg::DevBuf spill;  // traversal stack overflow, sized per launch (grow-only)
+  ztg::DevBuf bands;  // per-XCD tile counters of the persistent megakernel
   ztg::WfBuffers wf;
   std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };

@@ -13,6 +13,9 @@ void checkErr(zt_ctx* ctx);
 void uploadMaterialsAndLights(DevScene& s, hipStream_t stream);
 void rebuildLights(DevScene& s, hipStream_t stream);
 
+// Blocks of `kernel` the device can hold at once (occupancy x CUs).
+uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds);
+
 // AUTO resolution: the autotuner's record for this render class, else the
 // megakernel defaults; explicit methods pass through (enums validated).
 TuneKey tuneKey(const DevScene& s, const zt_render_options& o, uint32_t method, uint32_t precision);

@@ -27,31 +27,24 @@ ZT_D HitRec<R> rayQuery(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
   return query<R>(sc, stk, o, d, R(kRayEpsilon), tmax, self, excl, anyhit, overflow);
 }
 
-// Maps block b to a pixel tile so that the blocks the hardware places on one
-// XCD (b mod 8) cover one contiguous band of tiles, sharing that XCD's L2.
-ZT_D uint32_t xcdTile(uint32_t b, uint32_t nblocks) {
-  const uint32_t q = nblocks / 8, r = nblocks % 8;
-  const uint32_t x = b % 8, i = b / 8;
-  return x * q + min(x, r) + i;
+// Tiles are split into 8 contiguous bands, one per XCD: the hardware places
+// block b on XCD b mod 8, so a block that works through band b mod 8 shares
+// its L2 with the blocks rendering the neighbouring tiles.
+ZT_D uint32_t bandStart(uint32_t band, uint32_t ntiles) {
+  const uint32_t q = ntiles / 8, r = ntiles % 8;
+  return band * q + min(band, r);
 }
 
+// One pixel: all its samples from its own stream, in order (render.zig:133-144).
 template <typename R>
-ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict__ out, uint32_t tile_w,
-                         uint32_t tile_h, uint32_t tiles_x, uint32_t ntiles, int cap, int* err) {
-  extern __shared__ int lds[];
-  const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack(lds, cap);
-  const uint32_t tile = xcdTile(blockIdx.x, ntiles);
-  const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
-  const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
-  if (x >= pr.width || y >= pr.height) return;
+ZT_D void renderPixel(const SceneView<R>& sc, const Params& pr, const LaneStack& stk, uint32_t x, uint32_t y,
+                      R* __restrict__ out, int* overflow) {
   const uint64_t pix = (uint64_t)y * pr.width + x;
   Rng rng = rngForStream(pr.seed, pix);
   V3<R> sum = splat(R(0));
   Path<R> st;
   uint32_t sample = 0;
   int phase = kCamera;
-  int overflow = 0;
   for (;;) {
     if (phase == kCamera) {
       if (sample == pr.spp) break;
@@ -60,7 +53,7 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
     if (phase == kExtend || phase == kShadow) {
       const bool sh = phase == kShadow;
       const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
-                                   sh ? st.sh_excl : -1, sh, &overflow);
+                                      sh ? st.sh_excl : -1, sh, overflow);
       phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
     }
     if (phase == kFinish) {
@@ -73,6 +66,36 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
   out[3 * pix + 0] = sum.x * inv_spp;
   out[3 * pix + 1] = sum.y * inv_spp;
   out[3 * pix + 2] = sum.z * inv_spp;
+}
+
+// Persistent megakernel: each block takes tile after tile from its XCD's band
+// (one atomic per tile on band_next[band]) and, once that band is done, from
+// the other bands in turn. A pixel's result does not depend on which block
+// renders it.
+template <typename R>
+ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict__ out, uint32_t tile_w,
+                         uint32_t tile_h, uint32_t tiles_x, uint32_t ntiles, int cap, int* spill,
+                         uint32_t* band_next, int* err) {
+  extern __shared__ int lds[];
+  __shared__ uint32_t s_tile;
+  const uint32_t tid = threadIdx.x;
+  const LaneStack stk = laneStack(lds, cap, spill);
+  int overflow = 0;
+  const uint32_t home = blockIdx.x % 8;
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t band = (home + k) % 8;
+    const uint32_t hi = bandStart(band + 1, ntiles);
+    for (;;) {
+      if (tid == 0) s_tile = bandStart(band, ntiles) + atomicAdd(&band_next[band], 1u);
+      __syncthreads();
+      const uint32_t tile = s_tile;
+      __syncthreads();
+      if (tile >= hi) break;
+      const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
+      const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
+      if (x < pr.width && y < pr.height) renderPixel<R>(sc, pr, stk, x, y, out, &overflow);
+    }
+  }
   if (overflow) atomicOr(err, 1);
 }
 
@@ -82,19 +105,18 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
 template <typename R, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1, 8))) void megakernel(
     SceneView<R> sc, Params pr, R* __restrict__ out, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
-    uint32_t ntiles, int cap, int* err) {
-  megakernelBody<R>(sc, pr, out, tile_w, tile_h, tiles_x, ntiles, cap, err);
+    uint32_t ntiles, int cap, int* spill, uint32_t* band_next, int* err) {
+  megakernelBody<R>(sc, pr, out, tile_w, tile_h, tiles_x, ntiles, cap, spill, band_next, err);
+}
+
+__global__ void clearBands(uint32_t* band_next) {
+  if (threadIdx.x < 8) band_next[threadIdx.x] = 0;
 }
 
 // traceWith for caller-given rays and RNG states (render.zig:61-102).
 template <typename R>
-__global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, const zt_ray* rays, uint64_t n,
-                                                   uint64_t* states, double* out, int* err) {
-  extern __shared__ int lds[];
-  const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack(lds, kLds);
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
-  if (i >= n) return;
+ZT_D void traceOne(const SceneView<R>& sc, const Params& pr, const LaneStack& stk, const zt_ray* rays, uint64_t i,
+                   uint64_t* states, double* out, int* overflow) {
   Rng rng{states[4 * i], states[4 * i + 1], states[4 * i + 2], states[4 * i + 3]};
   Path<R> st;
   st.ro = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
@@ -108,11 +130,10 @@ __global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, c
   st.depth = 0;
   st.terminated = 0;
   int phase = pr.max_depth == 0 ? kFinish : kExtend;
-  int overflow = 0;
   while (phase != kFinish) {
     const bool sh = phase == kShadow;
     const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
-                                 sh ? st.sh_excl : -1, sh, &overflow);
+                                 sh ? st.sh_excl : -1, sh, overflow);
     phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
   }
   out[3 * i + 0] = st.radiance.x;
@@ -122,25 +143,28 @@ __global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, c
   states[4 * i + 1] = rng.s1;
   states[4 * i + 2] = rng.s2;
   states[4 * i + 3] = rng.s3;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, const zt_ray* rays, uint64_t n,
+                                                   uint64_t* states, double* out, int* spill, int* err) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack(lds, kLds, spill);
+  int overflow = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    traceOne<R>(sc, pr, stk, rays, i, states, out, &overflow);
   if (overflow) atomicOr(err, 1);
 }
 
 // Scene.intersect / brute force for caller-given rays (no self exclusion:
 // these rays do not start on a surface).
 template <typename R>
-__global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt_ray* rays, uint64_t n, R tmin, R tmax,
-                                                       int mode, zt_hit* out, int* err) {
-  extern __shared__ int lds[];
-  const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack(lds, kLds);
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid;
-  if (i >= n) return;
-  const V3<R> o = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
-  const V3<R> d = mk((R)rays[i].dir[0], (R)rays[i].dir[1], (R)rays[i].dir[2]);
-  int overflow = 0;
-  if (mode == 2) sc.nodes4 = nullptr;  // the two-child traversal
+ZT_D zt_hit intersectOne(const SceneView<R>& sc, const LaneStack& stk, const zt_ray& ray, R tmin, R tmax, int mode,
+                         int* overflow) {
+  const V3<R> o = mk((R)ray.origin[0], (R)ray.origin[1], (R)ray.origin[2]);
+  const V3<R> d = mk((R)ray.dir[0], (R)ray.dir[1], (R)ray.dir[2]);
   const HitRec<R> h = mode == 1 ? bruteForce<R>(sc, o, d, tmin, tmax, -1, -1, false)
-                                : query<R>(sc, stk, o, d, tmin, tmax, -1, -1, false, &overflow);
+                                : query<R>(sc, stk, o, d, tmin, tmax, -1, -1, false, overflow);
   zt_hit r{};
   r.hit = h.prim >= 0 ? 1 : 0;
   if (h.prim >= 0) {
@@ -156,7 +180,20 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
     r.material = (uint32_t)s.material;
     r.prim = h.prim;
   }
-  out[i] = r;
+  return r;
+}
+
+// Scene.intersect / brute force for caller-given rays (no self exclusion:
+// these rays do not start on a surface).
+template <typename R>
+__global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt_ray* rays, uint64_t n, R tmin, R tmax,
+                                                       int mode, zt_hit* out, int* spill, int* err) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack(lds, kLds, spill);
+  if (mode == 2) sc.nodes4 = nullptr;  // the two-child traversal
+  int overflow = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = intersectOne<R>(sc, stk, rays[i], tmin, tmax, mode, &overflow);
   if (overflow) atomicOr(err, 1);
 }
 
@@ -174,6 +211,8 @@ __global__ void rngKernel(uint64_t seed, const uint64_t* streams, uint64_t n, ui
 }
 
 static size_t ldsBytes(uint32_t threads) { return stackBytes(kLds, (int)threads); }
+// Grid cap of the caller-given-ray kernels (grid-stride loops).
+static constexpr uint64_t kRayGridBlocks = 2048;
 
 Params makeParams(const DevScene& s, const zt_render_options& o) {
   Params p{};
@@ -194,6 +233,14 @@ void checkOptions(const zt_render_options* o) {
   if (o->width > (1u << 20) || o->height > (1u << 20) || o->width * o->height > (1ull << 31) ||
       o->spp > 0xFFFFFFFFull)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "image size or spp beyond the device limits");
+}
+
+uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds) {
+  int dev = 0, cus = 0, per = 0;
+  ZT_HIP(hipGetDevice(&dev));
+  ZT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  ZT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)threads, lds));
+  return (uint32_t)std::max(1, cus * std::max(per, 1));
 }
 
 template <typename R>
@@ -219,28 +266,41 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   const size_t lds = stackBytes(cap, (int)threads);
   int* err = ctx->errflag.as<int>();
   const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
+  const void* fn = nullptr;
+  switch (variant) {
+    case 1: fn = (const void*)megakernel<R, 0>; break;
+    case 2: fn = (const void*)megakernel<R, 4>; break;
+    case 3: fn = (const void*)megakernel<R, 5>; break;
+    case 4: fn = (const void*)megakernel<R, 6>; break;
+    default: throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+  }
+  // Persistent grid: as many blocks as can be resident (never more than tiles).
+  const uint32_t grid = std::min<uint32_t>(ntiles, residentBlocks(fn, threads, lds));
+  ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
+  ctx->bands.ensure(8 * sizeof(uint32_t));
+  uint32_t* bands = ctx->bands.as<uint32_t>();
+  int* spill = ctx->spill.as<int>();
+  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
   switch (variant) {
     case 1:
-      hipLaunchKernelGGL((megakernel<R, 0>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, err);
+      hipLaunchKernelGGL((megakernel<R, 0>), dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, spill, bands, err);
       break;
     case 2:
-      hipLaunchKernelGGL((megakernel<R, 4>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, err);
+      hipLaunchKernelGGL((megakernel<R, 4>), dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, spill, bands, err);
       break;
     case 3:
-      hipLaunchKernelGGL((megakernel<R, 5>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, err);
-      break;
-    case 4:
-      hipLaunchKernelGGL((megakernel<R, 6>), dim3(ntiles), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, err);
+      hipLaunchKernelGGL((megakernel<R, 5>), dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, spill, bands, err);
       break;
     default:
-      throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+      hipLaunchKernelGGL((megakernel<R, 6>), dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, spill, bands, err);
+      break;
   }
   ZT_HIP(hipGetLastError());
-  ctx->stats.launches += 1;
+  ctx->stats.launches += 2;
 }
 template void launchMegakernel<float>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
                                       float*);
@@ -320,15 +380,16 @@ int zt_gpu_trace_rays(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64
     ZT_HIP(hipMemcpyAsync(dstates.p, rng_states, n * 32, hipMemcpyHostToDevice, ctx->stream));
     resetErr(ctx);
     const uint32_t threads = 256;
-    const uint32_t blocks = (uint32_t)((n + threads - 1) / threads);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + threads - 1) / threads, kRayGridBlocks);
+    ctx->spill.ensure(spillBytes((uint64_t)blocks * threads));
     if (precision == ZT_PRECISION_FP64)
       hipLaunchKernelGGL(traceKernel<double>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
                          viewOf<double>(s), pr, drays.as<zt_ray>(), n, dstates.as<uint64_t>(), dout.as<double>(),
-                         ctx->errflag.as<int>());
+                         ctx->spill.as<int>(), ctx->errflag.as<int>());
     else
       hipLaunchKernelGGL(traceKernel<float>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
                          viewOf<float>(s), pr, drays.as<zt_ray>(), n, dstates.as<uint64_t>(), dout.as<double>(),
-                         ctx->errflag.as<int>());
+                         ctx->spill.as<int>(), ctx->errflag.as<int>());
     ZT_HIP(hipGetLastError());
     ZT_HIP(hipMemcpyAsync(out_rgb, dout.p, n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ZT_HIP(hipMemcpyAsync(rng_states, dstates.p, n * 32, hipMemcpyDeviceToHost, ctx->stream));
@@ -351,15 +412,16 @@ int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_
     ZT_HIP(hipMemcpyAsync(drays.p, rays, n * sizeof(zt_ray), hipMemcpyHostToDevice, ctx->stream));
     resetErr(ctx);
     const uint32_t threads = 256;
-    const uint32_t blocks = (uint32_t)((n + threads - 1) / threads);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + threads - 1) / threads, kRayGridBlocks);
+    ctx->spill.ensure(spillBytes((uint64_t)blocks * threads));
     if (precision == ZT_PRECISION_FP64)
       hipLaunchKernelGGL(intersectKernel<double>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
                          viewOf<double>(s), drays.as<zt_ray>(), n, t_min, t_max, mode, dout.as<zt_hit>(),
-                         ctx->errflag.as<int>());
+                         ctx->spill.as<int>(), ctx->errflag.as<int>());
     else
       hipLaunchKernelGGL(intersectKernel<float>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
                          viewOf<float>(s), drays.as<zt_ray>(), n, (float)t_min, (float)t_max, mode,
-                         dout.as<zt_hit>(), ctx->errflag.as<int>());
+                         dout.as<zt_hit>(), ctx->spill.as<int>(), ctx->errflag.as<int>());
     ZT_HIP(hipGetLastError());
     ZT_HIP(hipMemcpyAsync(out, dout.p, n * sizeof(zt_hit), hipMemcpyDeviceToHost, ctx->stream));
     checkErr(ctx);

@@ -43,6 +43,7 @@ struct WfShared {
   uint32_t* qsh;      // shadow queue
   // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64)
   uint32_t* counters;
+  int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
   uint32_t num_slots;
   uint64_t total_pixels;
@@ -143,7 +144,7 @@ __global__ void wfInit(WfShared<R> w) {
 template <typename R>
 __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shadow, int cap) {
   extern __shared__ int lds[];
-  const LaneStack stk = laneStack(lds, cap);
+  const LaneStack stk = laneStack(lds, cap, w.spill);
   if (blockIdx.x == 0 && threadIdx.x == 0 && !shadow) {
     w.counters[1 - cur] = 0;  // next extension queue
     w.counters[2] = 0;        // shadow queue
@@ -233,11 +234,7 @@ __global__ void wfReset(uint32_t* counters, uint64_t first_free_pixel) {
 
 template <typename R>
 static int gridFor(const void* kernel, uint32_t threads, size_t lds) {
-  int dev = 0, cus = 0, per = 0;
-  ZT_HIP(hipGetDevice(&dev));
-  ZT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  ZT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)threads, lds));
-  return std::max(1, cus * std::max(per, 1));
+  return (int)residentBlocks(kernel, threads, lds);
 }
 
 template <typename R>
@@ -284,6 +281,8 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   const int g_shade = gridFor<R>((const void*)wfShade<R>, threads, 0);
   const int g_res = gridFor<R>((const void*)wfResolve<R>, threads, 0);
   const int g_init = std::max(1, (int)std::min<uint64_t>((nslots + threads - 1) / threads, 65535));
+  ctx->spill.ensure(spillBytes((uint64_t)g_tr * threads));
+  w.spill = ctx->spill.as<int>();
 
   hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, ctx->stream, w.counters, (uint64_t)nslots);
   hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, ctx->stream, w);
