@@ -64,8 +64,9 @@ enum { ZT_PRECISION_FP32 = 0, ZT_PRECISION_FP64 = 1 };
 typedef struct zt_gpu_tuning {
   uint32_t method;      /* ZT_METHOD_* */
   uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
-  uint32_t tile_w;      /* megakernel: pixels per workgroup row (0 = 16) */
-  uint32_t tile_h;      /* megakernel: pixel rows per workgroup (0 = 16) */
+  uint32_t tile_w;      /* megakernel: tile width in pixels (0 = 16); one workgroup per tile, except that the
+                             wave-scheduled kernel hands a tile (64..4096 pixels) to one wave at a time */
+  uint32_t tile_h;      /* megakernel: tile height in pixels (0 = 16) */
   uint32_t block_size;    /* wavefront: threads per workgroup (0 = 256) */
   uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
   uint32_t variant;       /* megakernel build: 0 default (fp32: 4, fp64: 1), 1 = registers unconstrained,
@@ -74,6 +75,11 @@ typedef struct zt_gpu_tuning {
                              (0 = megakernel fp32: 8, fp64: 16; wavefront: 16) */
   uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
                              nodes (the fp64 path always walks the two-child nodes) */
+  uint32_t schedule;      /* megakernel, fp32 over four-wide nodes: 0 or 2 = wave-scheduled (lanes shade and
+                             take new rays or pixels while other lanes keep traversing), 1 = one pixel per
+                             lane, every lane's ray finished before any lane shades */
+  uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
+                             1..64 (0 = 32) */
   uint32_t _reserved;
 } zt_gpu_tuning;
 

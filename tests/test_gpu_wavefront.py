@@ -66,6 +66,32 @@ def test_kernel_variants_and_stack_sizes_are_bit_identical(precision):
         assert np.array_equal(base, img), cap
 
 
+def test_wave_scheduled_megakernel_is_bit_identical():
+    """The wave-scheduled megakernel (lanes shade and refill while others traverse)
+    renders exactly what the one-pixel-per-lane loop renders, for any refill
+    threshold, tile shape (including ragged tiles and tiles larger than a wave)
+    and build variant; also with light sampling off and max_depth 0."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=3, width=70, height=37)
+    s, o = wl.scene, wl.options
+    base = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1))
+    for refill in (1, 17, 32, 64):
+        for tw, th in ((8, 8), (16, 16), (13, 5), (64, 64)):
+            img = zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, refill=refill, tile_w=tw, tile_h=th))
+            assert np.array_equal(base, img), (refill, tw, th)
+    for variant in (1, 2, 3, 4):
+        img = zt.render(s, o, zt.Tuning(method=MEGA, variant=variant, stack_entries=8))
+        assert np.array_equal(base, img), variant
+    for opts in (zt.Options(33, 20, 2, 12, light_sampling=False, seed=4), zt.Options(9, 7, 2, 0)):
+        a = zt.render(s, opts, zt.Tuning(method=MEGA, schedule=1))
+        b = zt.render(s, opts, zt.Tuning(method=MEGA, schedule=2, refill=24))
+        assert np.array_equal(a, b)
+    with pytest.raises(zt.ZTError):
+        zt.render(s, o, zt.Tuning(method=MEGA, refill=65))
+    with pytest.raises(zt.ZTError):
+        zt.render(s, o, zt.Tuning(method=MEGA, tile_w=2, tile_h=2))
+
+
 def test_two_child_and_four_wide_traversal_agree():
     """fp32 renders walk the four-wide BVH by default; the two-child walk finds the
     same closest primitives (exact t ties aside), so the images agree; fp64 always

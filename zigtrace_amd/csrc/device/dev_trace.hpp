@@ -293,6 +293,18 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
   return HitRec<R>{closest, best};
 }
 
+// Compare-exchange of (entry distance, child code) pairs: after it, a holds
+// the nearer entry.
+ZT_D void sortChild(float& ka, int& ca, float& kb, int& cb) {
+  const bool sw = ka > kb;
+  const float k = sw ? kb : ka;
+  const int c = sw ? cb : ca;
+  kb = sw ? ka : kb;
+  cb = sw ? ca : cb;
+  ka = k;
+  ca = c;
+}
+
 // Four-wide traversal of the fp32 product path over the same fp32 boxes and
 // primitive tests as `traverse`: a node's hit children are ordered by entry
 // distance with a sorting network, the farther ones pushed and the nearest
@@ -328,15 +340,6 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
     stk.tval[sp * stk.stride] = t;
     ++sp;
   };
-  auto cswap = [](float& ka, int& ca, float& kb, int& cb) {
-    const bool sw = ka > kb;
-    const float k = sw ? kb : ka;
-    const int c = sw ? cb : ca;
-    kb = sw ? ka : kb;
-    cb = sw ? ca : cb;
-    ka = k;
-    ca = c;
-  };
   int cur = 0;  // root
   for (;;) {
     if (cur == kNoChild) {
@@ -368,11 +371,11 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
       if (!(h1 && c1 != kNoChild)) k1 = INFINITY, c1 = kNoChild;
       if (!(h2 && c2 != kNoChild)) k2 = INFINITY, c2 = kNoChild;
       if (!(h3 && c3 != kNoChild)) k3 = INFINITY, c3 = kNoChild;
-      cswap(k0, c0, k1, c1);
-      cswap(k2, c2, k3, c3);
-      cswap(k0, c0, k2, c2);
-      cswap(k1, c1, k3, c3);
-      cswap(k1, c1, k2, c2);
+      sortChild(k0, c0, k1, c1);
+      sortChild(k2, c2, k3, c3);
+      sortChild(k0, c0, k2, c2);
+      sortChild(k1, c1, k3, c3);
+      sortChild(k1, c1, k2, c2);
       if (c3 != kNoChild) push(c3, k3);
       if (c2 != kNoChild) push(c2, k2);
       if (c1 != kNoChild) push(c1, k1);
@@ -428,6 +431,110 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
   }
   return HitRec<R>{closest, best};
 }
+
+// `traverse4` as a resumable walk whose state lives in registers, for
+// kernels that interleave many rays per lane: `start` sets up a query, each
+// `step` does one unit of work (a pop, a node or a leaf), and `step` returns
+// true once the BVH part is finished; `finish` then adds the planes and
+// returns the hit. The sequence of stack operations, box tests and primitive
+// tests is exactly traverse4's, so the hit is identical.
+struct Trav4 {
+  V3<float> o, d, inv;
+  float tmin, closest;
+  int best, self, excl, cur, sp, nsp;
+  bool nx, ny, nz, anyhit, found;
+
+  ZT_D void start(V3<float> o_, V3<float> d_, float tmin_, float tmax, int self_, int excl_, bool anyhit_) {
+    o = o_;
+    d = d_;
+    inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    nx = inv.x < 0;
+    ny = inv.y < 0;
+    nz = inv.z < 0;
+    tmin = tmin_;
+    closest = tmax;
+    best = -1;
+    self = self_;
+    excl = excl_;
+    anyhit = anyhit_;
+    found = false;
+    cur = 0;  // root
+    sp = 0;
+    nsp = 0;
+  }
+  ZT_D void push(const LaneStack& stk, int c, float t, int* overflow) {
+    const int half = stk.cap >> 1;
+    if (sp == stk.cap) {
+      if (nsp + half > kSpill) {
+        *overflow = 1;
+        return;
+      }
+      for (int i = 0; i < half; ++i) {
+        stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
+        stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
+        stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
+        stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
+      }
+      nsp += half;
+      sp = half;
+    }
+    stk.code[sp * stk.stride] = c;
+    stk.tval[sp * stk.stride] = t;
+    ++sp;
+  }
+  ZT_D bool step(const SceneView<float>& sc, const LaneStack& stk, int* overflow) {
+    if (cur == kNoChild) {
+      if (sp == 0) {
+        if (nsp == 0) return true;
+        const int half = stk.cap >> 1;
+        nsp -= half;
+        for (int i = 0; i < half; ++i) {
+          stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
+          stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
+        }
+        sp = half;
+      }
+      --sp;
+      if (stk.tval[sp * stk.stride] > closest) return false;
+      cur = stk.code[sp * stk.stride];
+      return false;
+    }
+    if (cur >= 0) {
+      const DNode4* n = sc.nodes4 + cur;
+      const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
+      int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
+      float k0, k1, k2, k3;
+      const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
+      const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
+      const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
+      const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
+      if (!(h0 && c0 != kNoChild)) k0 = INFINITY, c0 = kNoChild;
+      if (!(h1 && c1 != kNoChild)) k1 = INFINITY, c1 = kNoChild;
+      if (!(h2 && c2 != kNoChild)) k2 = INFINITY, c2 = kNoChild;
+      if (!(h3 && c3 != kNoChild)) k3 = INFINITY, c3 = kNoChild;
+      sortChild(k0, c0, k1, c1);
+      sortChild(k2, c2, k3, c3);
+      sortChild(k0, c0, k2, c2);
+      sortChild(k1, c1, k3, c3);
+      sortChild(k1, c1, k2, c2);
+      if (c3 != kNoChild) push(stk, c3, k3, overflow);
+      if (c2 != kNoChild) push(stk, c2, k2, overflow);
+      if (c1 != kNoChild) push(stk, c1, k1, overflow);
+      cur = c0;
+      return false;
+    }
+    if (leafHits<float>(sc, cur, o, d, tmin, &closest, &best, self, excl, anyhit)) {
+      found = true;
+      return true;
+    }
+    cur = kNoChild;
+    return false;
+  }
+  ZT_D HitRec<float> finish(const SceneView<float>& sc) {
+    if (!found) planeHits<float>(sc, o, d, tmin, &closest, &best, self, excl, anyhit);
+    return HitRec<float>{closest, best};
+  }
+};
 
 // One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force
 // when the scene has no BVH, the four-wide BVH on the fp32 path when the view

@@ -15,7 +15,7 @@
 #include "zt_gpu.h"
 
 // The ABI records keep the sizes the bindings are written against.
-static_assert(sizeof(zt_gpu_tuning) == 40, "zt_gpu_tuning layout");
+static_assert(sizeof(zt_gpu_tuning) == 48, "zt_gpu_tuning layout");
 static_assert(sizeof(zt_gpu_stats) == 48, "zt_gpu_stats layout");
 static_assert(sizeof(zt_render_options) == 64, "zt_render_options layout");
 static_assert(sizeof(zt_ray) == 48 && sizeof(zt_hit) == 72, "zt_ray / zt_hit layout");

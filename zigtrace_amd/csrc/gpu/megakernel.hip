@@ -109,6 +109,129 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
   megakernelBody<R>(sc, pr, out, tile_w, tile_h, tiles_x, ntiles, cap, spill, band_next, err);
 }
 
+// Wave-scheduled megakernel of the fp32 path over the four-wide BVH. One
+// wave per workgroup; every lane owns a pixel (its stream, accumulator and
+// sample count, as renderPixel) and at most one ray in flight, walked with
+// the resumable Trav4. The wave keeps stepping traversals until `refill`
+// lanes are waiting for work, then those lanes shade their hit (or apply
+// their shadow ray), start their next ray or sample, or take the next pixel
+// of the wave's current tile, while the other lanes keep their traversal
+// state. Each pixel's computation is the same sequence of device functions
+// as renderPixel's, so the image is identical; only the interleaving of
+// lanes changes (refill 64 = every lane finishes its ray before any shades).
+template <int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
+    SceneView<float> sc, Params pr, float* __restrict__ out, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
+    uint32_t ntiles, int cap, uint32_t refill, int* spill, uint32_t* band_next, int* err) {
+  constexpr int kResolve = 4;  // a traversal result is waiting to be applied
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack(lds, cap, spill);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t tile_px = tile_w * tile_h;
+  const uint32_t home = blockIdx.x % 8;
+  // Wave-uniform tile cursor.
+  uint32_t band_k = 0, tile = 0, next_px = tile_px;
+  bool tiles_done = false;
+  // Lane state.
+  bool has = false, tracing = false, sh = false;
+  uint32_t x = 0, y = 0, sample = 0;
+  int phase = kCamera;
+  Rng rng{};
+  V3<float> sum = splat(0.0f);
+  Path<float> st;
+  Trav4 tr;
+  HitRec<float> hit{0.0f, -1};
+  int overflow = 0;
+  for (;;) {
+    // Pixels for lanes without one, from the wave's tile (then the next tile).
+    for (;;) {
+      const uint64_t need = __ballot(!has);
+      if (need == 0 || tiles_done) break;
+      if (next_px >= tile_px) {
+        for (;;) {
+          if (band_k == 8) {
+            tiles_done = true;
+            break;
+          }
+          const uint32_t band = (home + band_k) % 8;
+          uint32_t t = 0;
+          if (lane == 0) t = bandStart(band, ntiles) + atomicAdd(&band_next[band], 1u);
+          t = __shfl(t, 0);
+          if (t < bandStart(band + 1, ntiles)) {
+            tile = t;
+            next_px = 0;
+            break;
+          }
+          ++band_k;
+        }
+        if (tiles_done) break;
+      }
+      const uint32_t idx = next_px + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+      next_px = min(tile_px, next_px + (uint32_t)__popcll(need));
+      if (!has && idx < tile_px) {
+        const uint32_t px = (tile % tiles_x) * tile_w + idx % tile_w;
+        const uint32_t py = (tile / tiles_x) * tile_h + idx / tile_w;
+        if (px < pr.width && py < pr.height) {
+          x = px;
+          y = py;
+          has = true;
+          rng = rngForStream(pr.seed, (uint64_t)py * pr.width + px);  // render.zig:135
+          sum = splat(0.0f);
+          sample = 0;
+          phase = kCamera;
+        }
+      }
+    }
+    // Lanes with no ray in flight advance their path to its next ray.
+    if (has && !tracing) {
+      for (;;) {
+        if (phase == kResolve) phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc, pr, hit, rng, st);
+        if (phase == kFinish) {
+          if (isFinite(st.radiance)) sum = add(sum, st.radiance);  // render.zig:141-142
+          ++sample;
+          phase = kCamera;
+        }
+        if (phase == kCamera) {
+          if (sample == pr.spp) {
+            const uint64_t pix = (uint64_t)y * pr.width + x;
+            const float inv_spp = 1.0f / (float)pr.spp;
+            out[3 * pix + 0] = sum.x * inv_spp;
+            out[3 * pix + 1] = sum.y * inv_spp;
+            out[3 * pix + 2] = sum.z * inv_spp;
+            has = false;
+            break;
+          }
+          phase = beginPath<float>(sc, pr, x, y, rng, st);
+          continue;  // kFinish at once when max_depth == 0
+        }
+        sh = phase == kShadow;
+        tr.start(st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, sh ? st.sh_tmax : INFINITY, st.self,
+                 sh ? st.sh_excl : -1, sh);
+        tracing = true;
+        break;
+      }
+    }
+    if (__ballot(tracing) == 0) {
+      // Every lane is without a pixel (a lane with one always has a ray now).
+      if (tiles_done) break;
+      continue;
+    }
+    // Step traversals until enough lanes wait for work.
+    for (;;) {
+      if (tracing && tr.step(sc, stk, &overflow)) {
+        hit = tr.finish(sc);
+        tracing = false;
+        phase = kResolve;
+      }
+      const uint64_t busy = __ballot(tracing);
+      if (busy == 0) break;
+      const uint32_t waiting = (uint32_t)__popcll(__ballot(!tracing && (has || !tiles_done)));
+      if (waiting >= refill) break;
+    }
+  }
+  if (overflow) atomicOr(err, 1);
+}
+
 __global__ void clearBands(uint32_t* band_next) {
   if (threadIdx.x < 8) band_next[threadIdx.x] = 0;
 }
@@ -235,6 +358,55 @@ void checkOptions(const zt_render_options* o) {
     throw Error(ZT_ERR_INVALID_ARGUMENT, "image size or spp beyond the device limits");
 }
 
+// Wave-scheduled fp32 render over the four-wide BVH (see megakernelWave).
+static void launchWave(zt_ctx* ctx, const SceneView<float>& view, const Params& pr, float* dout,
+                       const zt_gpu_tuning& tu, uint32_t variant, int cap, int* err) {
+  const uint32_t tw = tu.tile_w ? tu.tile_w : 16;
+  const uint32_t th = tu.tile_h ? tu.tile_h : 16;
+  if (tw == 0 || th == 0 || tw * th < 64 || tw * th > 4096)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "wave-scheduled tiles must hold 64..4096 pixels");
+  const uint32_t refill = tu.refill ? tu.refill : 32;
+  if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
+  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
+  const uint32_t ntiles = tiles_x * ((pr.height + th - 1) / th);
+  const uint32_t threads = 64;
+  const size_t lds = stackBytes(cap, (int)threads);
+  const void* fn = nullptr;
+  switch (variant) {
+    case 1: fn = (const void*)megakernelWave<1>; break;
+    case 2: fn = (const void*)megakernelWave<4>; break;
+    case 3: fn = (const void*)megakernelWave<5>; break;
+    case 4: fn = (const void*)megakernelWave<6>; break;
+    default: throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+  }
+  const uint32_t grid = std::min<uint32_t>(ntiles * (tw * th + 63) / 64, residentBlocks(fn, threads, lds));
+  ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
+  ctx->bands.ensure(8 * sizeof(uint32_t));
+  uint32_t* bands = ctx->bands.as<uint32_t>();
+  int* spill = ctx->spill.as<int>();
+  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
+  switch (variant) {
+    case 1:
+      hipLaunchKernelGGL(megakernelWave<1>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, refill, spill, bands, err);
+      break;
+    case 2:
+      hipLaunchKernelGGL(megakernelWave<4>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, refill, spill, bands, err);
+      break;
+    case 3:
+      hipLaunchKernelGGL(megakernelWave<5>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, refill, spill, bands, err);
+      break;
+    default:
+      hipLaunchKernelGGL(megakernelWave<6>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
+                         tiles_x, ntiles, cap, refill, spill, bands, err);
+      break;
+  }
+  ZT_HIP(hipGetLastError());
+  ctx->stats.launches += 2;
+}
+
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds) {
   int dev = 0, cus = 0, per = 0;
   ZT_HIP(hipGetDevice(&dev));
@@ -253,6 +425,17 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   // fp32 compiled for 6 waves per SIMD with 8 LDS stack entries per lane (LDS
   // then allows 8 waves), fp64 unconstrained with 16 entries.
   constexpr bool kF64 = K<R>::exact;
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : (kF64 ? 16 : 8);
+  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  if (tu.schedule > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "schedule must be 0..2");
+  int* err = ctx->errflag.as<int>();
+  const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
+  if constexpr (!kF64) {
+    if (view.nodes4 && tu.schedule != 1) {
+      launchWave(ctx, view, pr, dout, tu, variant, cap, err);
+      return;
+    }
+  }
   uint32_t tw = tu.tile_w ? tu.tile_w : 16;
   uint32_t th = tu.tile_h ? tu.tile_h : 16;
   const uint32_t threads = tw * th;
@@ -261,11 +444,7 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;
   const uint32_t tiles_y = (pr.height + th - 1) / th;
   const uint32_t ntiles = tiles_x * tiles_y;
-  const int cap = tu.stack_entries ? (int)tu.stack_entries : (kF64 ? 16 : 8);
-  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
   const size_t lds = stackBytes(cap, (int)threads);
-  int* err = ctx->errflag.as<int>();
-  const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
   const void* fn = nullptr;
   switch (variant) {
     case 1: fn = (const void*)megakernel<R, 0>; break;

@@ -46,11 +46,14 @@ class Tuning:
     variant: int = 0
     stack_entries: int = 0
     bvh_width: int = 0
+    schedule: int = 0
+    refill: int = 0
 
     def record(self) -> np.ndarray:
         return N.record(N.TUNING, method=self.method, precision=self.precision, tile_w=self.tile_w,
                         tile_h=self.tile_h, block_size=self.block_size, queue_paths=self.queue_paths,
-                        variant=self.variant, stack_entries=self.stack_entries, bvh_width=self.bvh_width)
+                        variant=self.variant, stack_entries=self.stack_entries, bvh_width=self.bvh_width,
+                        schedule=self.schedule, refill=self.refill)
 
 
 def render(scene: Scene, opts: Options, tuning: Tuning | None = None) -> np.ndarray:
