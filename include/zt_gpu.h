@@ -64,9 +64,10 @@ enum { ZT_PRECISION_FP32 = 0, ZT_PRECISION_FP64 = 1 };
 typedef struct zt_gpu_tuning {
   uint32_t method;      /* ZT_METHOD_* */
   uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
-  uint32_t tile_w;      /* megakernel: tile width in pixels (0 = 16); one workgroup per tile, except that the
-                             wave-scheduled kernel hands a tile (64..4096 pixels) to one wave at a time */
-  uint32_t tile_h;      /* megakernel: tile height in pixels (0 = 16) */
+  uint32_t tile_w;      /* megakernel: tile width in pixels; one workgroup per tile (64..256 pixels, 0 = 16x16),
+                             except that the wave-scheduled kernel hands tiles of 64..4096 pixels to one wave at
+                             a time (0 = 8x8) */
+  uint32_t tile_h;      /* megakernel: tile height in pixels */
   uint32_t block_size;    /* wavefront: threads per workgroup (0 = 256) */
   uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
   uint32_t variant;       /* megakernel build: 0 default (fp32: 4, fp64: 1), 1 = registers unconstrained,

@@ -361,8 +361,10 @@ void checkOptions(const zt_render_options* o) {
 // Wave-scheduled fp32 render over the four-wide BVH (see megakernelWave).
 static void launchWave(zt_ctx* ctx, const SceneView<float>& view, const Params& pr, float* dout,
                        const zt_gpu_tuning& tu, uint32_t variant, int cap, int* err) {
-  const uint32_t tw = tu.tile_w ? tu.tile_w : 16;
-  const uint32_t th = tu.tile_h ? tu.tile_h : 16;
+  // A tile is a wave's unit of work: 8x8 gives every lane one pixel to start
+  // with and keeps enough tiles for every resident wave on small images.
+  const uint32_t tw = tu.tile_w ? tu.tile_w : 8;
+  const uint32_t th = tu.tile_h ? tu.tile_h : 8;
   if (tw == 0 || th == 0 || tw * th < 64 || tw * th > 4096)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "wave-scheduled tiles must hold 64..4096 pixels");
   const uint32_t refill = tu.refill ? tu.refill : 32;
@@ -379,7 +381,7 @@ static void launchWave(zt_ctx* ctx, const SceneView<float>& view, const Params& 
     case 4: fn = (const void*)megakernelWave<6>; break;
     default: throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
   }
-  const uint32_t grid = std::min<uint32_t>(ntiles * (tw * th + 63) / 64, residentBlocks(fn, threads, lds));
+  const uint32_t grid = std::min<uint32_t>(ntiles, residentBlocks(fn, threads, lds));
   ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
   ctx->bands.ensure(8 * sizeof(uint32_t));
   uint32_t* bands = ctx->bands.as<uint32_t>();
