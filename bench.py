@@ -69,11 +69,11 @@ def cpu_baseline(workload, target_seconds: float = 15.0):
         t = time.time()
         _, ctr = ref.render(o.width, o.height, o.spp, o.max_depth, threads=threads, seed=o.seed, pixel_stride=stride)
         dt = time.time() - t
-        if dt > target_seconds / 4 or stride <= 16:
+        if dt > target_seconds / 3 or stride == 1:
             break
         rate = int(ctr["paths"]) / max(dt, 1e-9)
         want_paths = rate * target_seconds
-        stride = max(16, int(npix * o.spp / max(want_paths, 1)))
+        stride = max(1, int(npix * o.spp / max(want_paths, 1)))
     paths = int(ctr["paths"])
     rays = int(ctr["extend_rays"]) + int(ctr["shadow_rays"])
     return {
@@ -81,6 +81,26 @@ def cpu_baseline(workload, target_seconds: float = 15.0):
         "sample": f"every {stride}th pixel of the {o.width}x{o.height} frame, all {o.spp} spp ({paths} paths, "
                   f"{dt:.1f}s), C++ f64 restatement of the reference CPU path",
     }, ctr, rays / max(paths, 1)
+
+
+def fixture_bytes_per_path():
+    """B_path of the mesh field from tests/golden/roofline_mesh_field.json (oracle counts)."""
+    with open(os.path.join(ROOT, "tests", "golden", "roofline_mesh_field.json")) as f:
+        fx = json.load(f)
+    return fx["bytes_per_path"], fx["rays_per_path"]
+
+
+def measured_traffic_per_path():
+    """HBM bytes per path of the megakernel on the mesh field, from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for a copy
+    kernel of known size); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "megakernel_mesh_field_1080p64_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_path:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def copy_probe(nbytes: int) -> int:
@@ -159,7 +179,7 @@ def bench_adjoint(zt, N, W, steps: int):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--method", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -212,7 +232,8 @@ def main() -> int:
             log(f"[bench] {name} step {i + 1}/{args.steps}: {kms[-1]:.1f} ms")
         wall = time.perf_counter() - t
         results[name] = {"ms_per_step": wall * 1e3 / args.steps, "kernel_ms": float(np.mean(kms)),
-                         "mpaths_s": paths * args.steps / wall / 1e6,
+                         "kernel_ms_median": float(np.median(kms)), "mpaths_s": paths * args.steps / wall / 1e6,
+                         "mpaths_s_median": paths / float(np.median(kms)) / 1e3,
                          "tuning": {k: v for k, v in tu.__dict__.items() if v}}
     img = ctx.to_host(dout, np.zeros((o.height, o.width, 3), np.float32))
     assert np.isfinite(img).all()
@@ -224,14 +245,23 @@ def main() -> int:
         cpu, ctr, rays_per_path = cpu_baseline(w)
         log(f"[bench] cpu baseline {cpu['value']:.3f} Mpaths/s on {cpu['cores']} threads")
     roofline = None
-    if ctr is not None and "megakernel" in results:
-        bpp = algorithmic_bytes_per_path(ctr, o.spp)
+    if "megakernel" in results:
+        if ctr is not None:
+            bpp, source = algorithmic_bytes_per_path(ctr, o.spp), "oracle counts on this run's CPU sample"
+        else:
+            bpp, rays_per_path = fixture_bytes_per_path()
+            source = "tests/golden/roofline_mesh_field.json"
+        if args.spp != 64:
+            bpp = bpp - 12.0 / 64 + 12.0 / o.spp
         km = results["megakernel"]["kernel_ms"]
         achieved = paths * bpp / (km * 1e-3) / 1e9
+        tpp = measured_traffic_per_path()
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": round(tpp * paths) if tpp is not None else None,
                     "kernel": "megakernel", "kernel_ms": round(km, 3), "bytes_per_path": round(bpp, 2),
-                    "bytes_per_launch": round(paths * bpp)}
+                    "bytes_per_launch": round(paths * bpp), "bytes_source": source,
+                    "traffic_source": "profiles/megakernel_mesh_field_1080p64_pmc.txt" if tpp is not None else None}
     extra = {}
     if not args.no_extra:
         ctx.free(dout)
@@ -241,7 +271,7 @@ def main() -> int:
         extra["batch"] = bench_batch(zt, N, W, steps=1)
         log(f"[bench] batch leg {time.time() - t:.1f}s: {extra['batch']}")
         t = time.time()
-        extra["adjoint"] = bench_adjoint(zt, N, W, steps=50)
+        extra["adjoint"] = bench_adjoint(zt, N, W, steps=200)
         log(f"[bench] adjoint leg {time.time() - t:.1f}s: {extra['adjoint']}")
     line = {
         "metric": "Mpaths/s @1080p 64spp (plain vs wavefront-batched); fwd+adjoint iters/s",
@@ -254,7 +284,7 @@ def main() -> int:
                    "light_sampling": True, "method": best},
         "methods": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
-        "mrays_s": round(r["mpaths_s"] * rays_per_path, 2) if rays_per_path else None,
+        "mrays_s": round(r["mpaths_s"] * (rays_per_path or fixture_bytes_per_path()[1]), 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -64,7 +64,7 @@ enum { ZT_PRECISION_FP32 = 0, ZT_PRECISION_FP64 = 1 };
 typedef struct zt_gpu_tuning {
   uint32_t method;      /* ZT_METHOD_* */
   uint32_t precision;   /* ZT_PRECISION_FP32 (product path) or _FP64 (verification path) */
-  uint32_t tile_w;      /* megakernel: tile width in pixels; one workgroup per tile (64..256 pixels, 0 = 16x16),
+  uint32_t tile_w;      /* megakernel: tile width in pixels; one workgroup per tile (64..256 pixels, 0 = 8x8),
                              except that the wave-scheduled kernel hands tiles of 64..4096 pixels to one wave at
                              a time (0 = 8x8) */
   uint32_t tile_h;      /* megakernel: tile height in pixels */
@@ -76,9 +76,9 @@ typedef struct zt_gpu_tuning {
                              (0 = megakernel fp32: 8, fp64: 16; wavefront: 16) */
   uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
                              nodes (the fp64 path always walks the two-child nodes) */
-  uint32_t schedule;      /* megakernel, fp32 over four-wide nodes: 0 or 2 = wave-scheduled (lanes shade and
-                             take new rays or pixels while other lanes keep traversing), 1 = one pixel per
-                             lane, every lane's ray finished before any lane shades */
+  uint32_t schedule;      /* megakernel: 0 or 1 = one pixel per lane, every lane's ray finished before any lane
+                             shades; 2 (fp32 over four-wide nodes) = wave-scheduled: lanes shade and take new
+                             rays or pixels while other lanes keep traversing */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
                              1..64 (0 = 32) */
   uint32_t _reserved;

@@ -80,16 +80,16 @@ def test_wave_scheduled_megakernel_is_bit_identical():
             img = zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, refill=refill, tile_w=tw, tile_h=th))
             assert np.array_equal(base, img), (refill, tw, th)
     for variant in (1, 2, 3, 4):
-        img = zt.render(s, o, zt.Tuning(method=MEGA, variant=variant, stack_entries=8))
+        img = zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, variant=variant, stack_entries=8))
         assert np.array_equal(base, img), variant
     for opts in (zt.Options(33, 20, 2, 12, light_sampling=False, seed=4), zt.Options(9, 7, 2, 0)):
         a = zt.render(s, opts, zt.Tuning(method=MEGA, schedule=1))
         b = zt.render(s, opts, zt.Tuning(method=MEGA, schedule=2, refill=24))
         assert np.array_equal(a, b)
     with pytest.raises(zt.ZTError):
-        zt.render(s, o, zt.Tuning(method=MEGA, refill=65))
+        zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, refill=65))
     with pytest.raises(zt.ZTError):
-        zt.render(s, o, zt.Tuning(method=MEGA, tile_w=2, tile_h=2))
+        zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_w=2, tile_h=2))
 
 
 def test_two_child_and_four_wide_traversal_agree():

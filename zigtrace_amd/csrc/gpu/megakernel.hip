@@ -423,9 +423,10 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);  // render.zig:157,162
   applyBvhWidth(view, tu);
   Params pr = makeParams(s, o);
-  // Defaults measured on the 1080p mesh field (1.3M triangles): 16x16 tiles;
-  // fp32 compiled for 6 waves per SIMD with 8 LDS stack entries per lane (LDS
-  // then allows 8 waves), fp64 unconstrained with 16 entries.
+  // Defaults measured on the 1080p mesh field (1.3M triangles): 8x8 tiles
+  // (one wave each; many tiles per resident block balance the tail); fp32
+  // compiled for 6 waves per SIMD with 8 LDS stack entries per lane, fp64
+  // unconstrained with 16 entries.
   constexpr bool kF64 = K<R>::exact;
   const int cap = tu.stack_entries ? (int)tu.stack_entries : (kF64 ? 16 : 8);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
@@ -433,13 +434,13 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   int* err = ctx->errflag.as<int>();
   const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
   if constexpr (!kF64) {
-    if (view.nodes4 && tu.schedule != 1) {
+    if (view.nodes4 && tu.schedule == 2) {
       launchWave(ctx, view, pr, dout, tu, variant, cap, err);
       return;
     }
   }
-  uint32_t tw = tu.tile_w ? tu.tile_w : 16;
-  uint32_t th = tu.tile_h ? tu.tile_h : 16;
+  uint32_t tw = tu.tile_w ? tu.tile_w : 8;
+  uint32_t th = tu.tile_h ? tu.tile_h : 8;
   const uint32_t threads = tw * th;
   if (threads == 0 || threads > 256 || threads % 64 != 0)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel tile must hold 64, 128, 192 or 256 pixels");
