@@ -340,24 +340,31 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
     stk.tval[sp * stk.stride] = t;
     ++sp;
   };
+  // While-while: a lane pops and visits interior nodes until it holds a leaf
+  // (or its stack is empty), then tests that leaf, so a wave runs node work
+  // and leaf work in separate passes instead of both in every iteration.
   int cur = 0;  // root
   for (;;) {
-    if (cur == kNoChild) {
-      if (sp == 0) {
-        if (nsp == 0) break;
-        nsp -= half;
-        for (int i = 0; i < half; ++i) {
-          stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
-          stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
+    bool empty = false;
+    while (cur >= 0 || cur == kNoChild) {
+      if (cur == kNoChild) {
+        if (sp == 0) {
+          if (nsp == 0) {
+            empty = true;
+            break;
+          }
+          nsp -= half;
+          for (int i = 0; i < half; ++i) {
+            stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
+            stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
+          }
+          sp = half;
         }
-        sp = half;
+        --sp;
+        if (stk.tval[sp * stk.stride] > closest) continue;
+        cur = stk.code[sp * stk.stride];
+        continue;
       }
-      --sp;
-      const float et = stk.tval[sp * stk.stride];
-      if (et > closest) continue;
-      cur = stk.code[sp * stk.stride];
-    }
-    if (cur >= 0) {
       const DNode4* n = sc.nodes4 + cur;
       const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
       int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
@@ -380,8 +387,8 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
       if (c2 != kNoChild) push(c2, k2);
       if (c1 != kNoChild) push(c1, k1);
       cur = c0;  // kNoChild when nothing was hit: pop next
-      continue;
     }
+    if (empty) break;
     if (leafHits<float>(sc, cur, o, d, tmin, &closest, &best, self, excl, anyhit)) {
       done = true;
       break;
