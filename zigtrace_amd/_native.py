@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libzigtrace_amd.so")
+# ZIGTRACE_AMD_LIB selects another build of the same library (e.g. an
+# instrumented or differently configured one); it must export the same ABI.
+LIB_PATH = os.environ.get("ZIGTRACE_AMD_LIB") or os.path.join(HERE, "lib", "libzigtrace_amd.so")
 
 # ---------------------------------------------------------------- status codes (zt_types.h)
 OK = 0
