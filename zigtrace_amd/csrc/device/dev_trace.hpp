@@ -305,100 +305,6 @@ ZT_D void sortChild(float& ka, int& ca, float& kb, int& cb) {
   ca = c;
 }
 
-// Four-wide traversal of the fp32 product path over the same fp32 boxes and
-// primitive tests as `traverse`: a node's hit children are ordered by entry
-// distance with a sorting network, the farther ones pushed and the nearest
-// visited next without a stack round trip; popped entries beyond the closest
-// hit are pruned. The closest hit is the primitive the two-child traversal
-// finds, except that two primitives at exactly the same t may be resolved the
-// other way (the visiting order differs).
-ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
-                             float tmax, int self, int excl, bool anyhit, int* overflow) {
-  float closest = tmax;
-  int best = -1;
-  bool done = false;
-  const V3<float> inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-  int sp = 0, nsp = 0;
-  const int half = stk.cap >> 1;
-  auto push = [&](int c, float t) {
-    if (sp == stk.cap) {
-      if (nsp + half > kSpill) {
-        *overflow = 1;
-        return;
-      }
-      for (int i = 0; i < half; ++i) {
-        stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
-        stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
-        stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
-        stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
-      }
-      nsp += half;
-      sp = half;
-    }
-    stk.code[sp * stk.stride] = c;
-    stk.tval[sp * stk.stride] = t;
-    ++sp;
-  };
-  // While-while: a lane pops and visits interior nodes until it holds a leaf
-  // (or its stack is empty), then tests that leaf, so a wave runs node work
-  // and leaf work in separate passes instead of both in every iteration.
-  int cur = 0;  // root
-  for (;;) {
-    bool empty = false;
-    while (cur >= 0 || cur == kNoChild) {
-      if (cur == kNoChild) {
-        if (sp == 0) {
-          if (nsp == 0) {
-            empty = true;
-            break;
-          }
-          nsp -= half;
-          for (int i = 0; i < half; ++i) {
-            stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
-            stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
-          }
-          sp = half;
-        }
-        --sp;
-        if (stk.tval[sp * stk.stride] > closest) continue;
-        cur = stk.code[sp * stk.stride];
-        continue;
-      }
-      const DNode4* n = sc.nodes4 + cur;
-      const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
-      int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
-      float k0, k1, k2, k3;
-      const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
-      const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
-      const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
-      const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
-      // Misses (and empty slots, whose boxes are inverted) sort last.
-      if (!(h0 && c0 != kNoChild)) k0 = INFINITY, c0 = kNoChild;
-      if (!(h1 && c1 != kNoChild)) k1 = INFINITY, c1 = kNoChild;
-      if (!(h2 && c2 != kNoChild)) k2 = INFINITY, c2 = kNoChild;
-      if (!(h3 && c3 != kNoChild)) k3 = INFINITY, c3 = kNoChild;
-      sortChild(k0, c0, k1, c1);
-      sortChild(k2, c2, k3, c3);
-      sortChild(k0, c0, k2, c2);
-      sortChild(k1, c1, k3, c3);
-      sortChild(k1, c1, k2, c2);
-      if (c3 != kNoChild) push(c3, k3);
-      if (c2 != kNoChild) push(c2, k2);
-      if (c1 != kNoChild) push(c1, k1);
-      cur = c0;  // kNoChild when nothing was hit: pop next
-    }
-    if (empty) break;
-    if (leafHits<float>(sc, cur, o, d, tmin, &closest, &best, self, excl, anyhit)) {
-      done = true;
-      break;
-    }
-    cur = kNoChild;
-  }
-  if (!done) planeHits<float>(sc, o, d, tmin, &closest, &best, self, excl, anyhit);
-  return HitRec<float>{closest, best};
-}
-
 // Brute force over every primitive in slot order, then planes
 // (geometry.zig:129-139 and scene.zig:109-117): the reference the BVH
 // traversal is checked against on the device, and the path for scenes
@@ -439,12 +345,16 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
   return HitRec<R>{closest, best};
 }
 
-// `traverse4` as a resumable walk whose state lives in registers, for
-// kernels that interleave many rays per lane: `start` sets up a query, each
-// `step` does one unit of work (a pop, a node or a leaf), and `step` returns
-// true once the BVH part is finished; `finish` then adds the planes and
-// returns the hit. The sequence of stack operations, box tests and primitive
-// tests is exactly traverse4's, so the hit is identical.
+// The four-wide walk as a resumable state machine whose state lives in
+// registers: `start` sets up a query; `inner` does one pop or one interior
+// node (the four slab tests, a 5-comparator sorting network on (entry
+// distance, code), the farther hits pushed farthest first, the nearest held
+// in `cur`); `leaf` tests the leaf held in `cur`; `finish` adds the planes
+// and returns the hit. Popped entries beyond the closest hit are dropped.
+// The closest hit is the primitive the two-child traversal finds, except
+// that two primitives at exactly the same t may be resolved the other way
+// (the visiting order differs). `traverse4` drives one query to the end;
+// the wave-scheduled megakernel interleaves many.
 struct Trav4 {
   V3<float> o, d, inv;
   float tmin, closest;
@@ -469,9 +379,14 @@ struct Trav4 {
     sp = 0;
     nsp = 0;
   }
-  ZT_D void push(const LaneStack& stk, int c, float t, int* overflow) {
-    const int half = stk.cap >> 1;
-    if (sp == stk.cap) {
+  ZT_D bool atLeaf() const { return cur < 0 && cur != kNoChild; }
+  // Pushes the m (0..3) farther hits, farthest first, so the nearest of them
+  // is popped first; one overflow check for the group.
+  ZT_D void push3(const LaneStack& stk, int m, int c1, float k1, int c2, float k2, int c3, float k3,
+                  int* overflow) {
+    if (m == 0) return;
+    if (sp + m > stk.cap) {
+      const int half = stk.cap >> 1;
       if (nsp + half > kSpill) {
         *overflow = 1;
         return;
@@ -479,17 +394,28 @@ struct Trav4 {
       for (int i = 0; i < half; ++i) {
         stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
         stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
-        stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
-        stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
+      }
+      for (int i = half; i < sp; ++i) {
+        stk.code[(i - half) * stk.stride] = stk.code[i * stk.stride];
+        stk.tval[(i - half) * stk.stride] = stk.tval[i * stk.stride];
       }
       nsp += half;
-      sp = half;
+      sp -= half;
     }
-    stk.code[sp * stk.stride] = c;
-    stk.tval[sp * stk.stride] = t;
-    ++sp;
+    stk.code[sp * stk.stride] = m == 3 ? c3 : (m == 2 ? c2 : c1);
+    stk.tval[sp * stk.stride] = m == 3 ? k3 : (m == 2 ? k2 : k1);
+    if (m >= 2) {
+      stk.code[(sp + 1) * stk.stride] = m == 3 ? c2 : c1;
+      stk.tval[(sp + 1) * stk.stride] = m == 3 ? k2 : k1;
+    }
+    if (m == 3) {
+      stk.code[(sp + 2) * stk.stride] = c1;
+      stk.tval[(sp + 2) * stk.stride] = k1;
+    }
+    sp += m;
   }
-  ZT_D bool step(const SceneView<float>& sc, const LaneStack& stk, int* overflow) {
+  // One pop or one interior node. Returns true when the BVH is exhausted.
+  ZT_D bool inner(const SceneView<float>& sc, const LaneStack& stk, int* overflow) {
     if (cur == kNoChild) {
       if (sp == 0) {
         if (nsp == 0) return true;
@@ -502,39 +428,42 @@ struct Trav4 {
         sp = half;
       }
       --sp;
-      if (stk.tval[sp * stk.stride] > closest) return false;
-      cur = stk.code[sp * stk.stride];
+      if (stk.tval[sp * stk.stride] <= closest) cur = stk.code[sp * stk.stride];
       return false;
     }
-    if (cur >= 0) {
-      const DNode4* n = sc.nodes4 + cur;
-      const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
-      int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
-      float k0, k1, k2, k3;
-      const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
-      const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
-      const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
-      const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
-      if (!(h0 && c0 != kNoChild)) k0 = INFINITY, c0 = kNoChild;
-      if (!(h1 && c1 != kNoChild)) k1 = INFINITY, c1 = kNoChild;
-      if (!(h2 && c2 != kNoChild)) k2 = INFINITY, c2 = kNoChild;
-      if (!(h3 && c3 != kNoChild)) k3 = INFINITY, c3 = kNoChild;
-      sortChild(k0, c0, k1, c1);
-      sortChild(k2, c2, k3, c3);
-      sortChild(k0, c0, k2, c2);
-      sortChild(k1, c1, k3, c3);
-      sortChild(k1, c1, k2, c2);
-      if (c3 != kNoChild) push(stk, c3, k3, overflow);
-      if (c2 != kNoChild) push(stk, c2, k2, overflow);
-      if (c1 != kNoChild) push(stk, c1, k1, overflow);
-      cur = c0;
-      return false;
-    }
-    if (leafHits<float>(sc, cur, o, d, tmin, &closest, &best, self, excl, anyhit)) {
+    const DNode4* n = sc.nodes4 + cur;
+    const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
+    int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
+    float k0, k1, k2, k3;
+    const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
+    const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
+    const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
+    const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
+    // Misses (and empty slots, whose boxes are inverted) sort last.
+    const bool v0 = h0 && c0 != kNoChild, v1 = h1 && c1 != kNoChild, v2 = h2 && c2 != kNoChild,
+               v3 = h3 && c3 != kNoChild;
+    if (!v0) k0 = INFINITY, c0 = kNoChild;
+    if (!v1) k1 = INFINITY, c1 = kNoChild;
+    if (!v2) k2 = INFINITY, c2 = kNoChild;
+    if (!v3) k3 = INFINITY, c3 = kNoChild;
+    sortChild(k0, c0, k1, c1);
+    sortChild(k2, c2, k3, c3);
+    sortChild(k0, c0, k2, c2);
+    sortChild(k1, c1, k3, c3);
+    sortChild(k1, c1, k2, c2);
+    const int hits = (int)v0 + (int)v1 + (int)v2 + (int)v3;
+    push3(stk, hits > 0 ? hits - 1 : 0, c1, k1, c2, k2, c3, k3, overflow);
+    cur = c0;  // kNoChild when nothing was hit: pop next
+    return false;
+  }
+  // Tests the held leaf. Returns true when an any-hit query found its hit.
+  ZT_D bool leaf(const SceneView<float>& sc) {
+    const int code = cur;
+    cur = kNoChild;
+    if (leafHits<float>(sc, code, o, d, tmin, &closest, &best, self, excl, anyhit)) {
       found = true;
       return true;
     }
-    cur = kNoChild;
     return false;
   }
   ZT_D HitRec<float> finish(const SceneView<float>& sc) {
@@ -542,6 +471,26 @@ struct Trav4 {
     return HitRec<float>{closest, best};
   }
 };
+
+// One four-wide query to the end, while-while: a lane pops and visits
+// interior nodes until it holds a leaf (or has nothing left), then tests that
+// leaf, so a wave runs node work and leaf work in separate passes.
+ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
+                             float tmax, int self, int excl, bool anyhit, int* overflow) {
+  Trav4 t;
+  t.start(o, d, tmin, tmax, self, excl, anyhit);
+  for (;;) {
+    bool empty = false;
+    while (!t.atLeaf()) {
+      if (t.inner(sc, stk, overflow)) {
+        empty = true;
+        break;
+      }
+    }
+    if (empty || t.leaf(sc)) break;
+  }
+  return t.finish(sc);
+}
 
 // One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force
 // when the scene has no BVH, the four-wide BVH on the fp32 path when the view

@@ -112,13 +112,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
 // Wave-scheduled megakernel of the fp32 path over the four-wide BVH. One
 // wave per workgroup; every lane owns a pixel (its stream, accumulator and
 // sample count, as renderPixel) and at most one ray in flight, walked with
-// the resumable Trav4. The wave keeps stepping traversals until `refill`
-// lanes are waiting for work, then those lanes shade their hit (or apply
-// their shadow ray), start their next ray or sample, or take the next pixel
-// of the wave's current tile, while the other lanes keep their traversal
-// state. Each pixel's computation is the same sequence of device functions
-// as renderPixel's, so the image is identical; only the interleaving of
-// lanes changes (refill 64 = every lane finishes its ray before any shades).
+// the resumable Trav4. The wave runs wave-uniform passes -- node steps, leaf
+// tests -- until `refill` lanes are waiting for work, then those lanes shade
+// their hit (or apply their shadow ray), start their next ray or sample, or
+// take the next pixel of the wave's current tile, while the other lanes keep
+// their traversal state. Each pixel's computation is the same sequence of
+// device functions as renderPixel's, so the image is identical; only the
+// interleaving of lanes changes.
 template <int W>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
     SceneView<float> sc, Params pr, float* __restrict__ out, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
@@ -216,17 +216,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       if (tiles_done) break;
       continue;
     }
-    // Step traversals until enough lanes wait for work.
+    // Traverse in wave-uniform passes until enough lanes wait for work: a
+    // leaf pass once at least as many lanes hold an untested leaf as still
+    // walk interior nodes, otherwise a node pass (one pop or node each), so
+    // no pass pays for the other's code.
     for (;;) {
-      if (tracing && tr.step(sc, stk, &overflow)) {
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      const uint32_t waiting = (uint32_t)__popcll(__ballot(!tracing && (has || !tiles_done)));
+      if (waiting >= refill) break;
+      bool ended = false;
+      if (n_leaf >= n_node) {
+        if (at_leaf) ended = tr.leaf(sc);
+      } else if (tracing && !at_leaf) {
+        ended = tr.inner(sc, stk, &overflow);
+      }
+      if (ended) {
         hit = tr.finish(sc);
         tracing = false;
         phase = kResolve;
       }
-      const uint64_t busy = __ballot(tracing);
-      if (busy == 0) break;
-      const uint32_t waiting = (uint32_t)__popcll(__ballot(!tracing && (has || !tiles_done)));
-      if (waiting >= refill) break;
     }
   }
   if (overflow) atomicOr(err, 1);
