@@ -187,6 +187,9 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner")
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel (64 is the benchmark; less for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--schedule", type=int, default=0, help="megakernel schedule with --no-tune (0/1 pixel, 2 wave)")
+    ap.add_argument("--refill", type=int, default=0, help="wave-scheduled refill threshold with --no-tune")
+    ap.add_argument("--stack", type=int, default=0, help="LDS stack entries with --no-tune")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
@@ -216,7 +219,9 @@ def main() -> int:
     chosen = list(methods) if args.method == "auto" else [args.method]
     results = {}
     for name in chosen:
-        tu = zt.Tuning(method=methods[name])
+        tu = zt.Tuning(method=methods[name], stack_entries=args.stack)
+        if name == "megakernel":
+            tu.schedule, tu.refill = args.schedule, args.refill
         if not args.no_tune:
             t = time.time()
             tu, probe_ms = ctx.autotune(sid, o, method=methods[name])

@@ -355,25 +355,26 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
 // that two primitives at exactly the same t may be resolved the other way
 // (the visiting order differs). `traverse4` drives one query to the end;
 // the wave-scheduled megakernel interleaves many.
-struct Trav4 {
-  V3<float> o, d, inv;
-  float tmin, closest;
-  int best, self, excl, cur, sp, nsp;
-  bool nx, ny, nz, anyhit, found;
+// The query a walk answers; kept by the caller (the wave-scheduled kernel
+// rebuilds it from the path state), so the walk itself holds only what
+// changes.
+struct Query4 {
+  V3<float> o, d;
+  float tmin;
+  int self, excl;
+  bool anyhit;
+};
 
-  ZT_D void start(V3<float> o_, V3<float> d_, float tmin_, float tmax, int self_, int excl_, bool anyhit_) {
-    o = o_;
-    d = d_;
-    inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    nx = inv.x < 0;
-    ny = inv.y < 0;
-    nz = inv.z < 0;
-    tmin = tmin_;
+struct Trav4 {
+  V3<float> inv;
+  float closest;
+  int best, cur, sp, nsp;
+  bool found;
+
+  ZT_D void start(const Query4& q, float tmax) {
+    inv = mk(1.0f / q.d.x, 1.0f / q.d.y, 1.0f / q.d.z);
     closest = tmax;
     best = -1;
-    self = self_;
-    excl = excl_;
-    anyhit = anyhit_;
     found = false;
     cur = 0;  // root
     sp = 0;
@@ -415,7 +416,7 @@ struct Trav4 {
     sp += m;
   }
   // One pop or one interior node. Returns true when the BVH is exhausted.
-  ZT_D bool inner(const SceneView<float>& sc, const LaneStack& stk, int* overflow) {
+  ZT_D bool inner(const SceneView<float>& sc, const LaneStack& stk, const Query4& q, int* overflow) {
     if (cur == kNoChild) {
       if (sp == 0) {
         if (nsp == 0) return true;
@@ -435,10 +436,11 @@ struct Trav4 {
     const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
     int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
     float k0, k1, k2, k3;
-    const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, nx, ny, nz, tmin, closest, &k0);
-    const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, nx, ny, nz, tmin, closest, &k1);
-    const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, nx, ny, nz, tmin, closest, &k2);
-    const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, nx, ny, nz, tmin, closest, &k3);
+    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, q.o, inv, nx, ny, nz, q.tmin, closest, &k0);
+    const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, q.o, inv, nx, ny, nz, q.tmin, closest, &k1);
+    const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, q.o, inv, nx, ny, nz, q.tmin, closest, &k2);
+    const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, q.o, inv, nx, ny, nz, q.tmin, closest, &k3);
     // Misses (and empty slots, whose boxes are inverted) sort last.
     const bool v0 = h0 && c0 != kNoChild, v1 = h1 && c1 != kNoChild, v2 = h2 && c2 != kNoChild,
                v3 = h3 && c3 != kNoChild;
@@ -457,17 +459,17 @@ struct Trav4 {
     return false;
   }
   // Tests the held leaf. Returns true when an any-hit query found its hit.
-  ZT_D bool leaf(const SceneView<float>& sc) {
+  ZT_D bool leaf(const SceneView<float>& sc, const Query4& q) {
     const int code = cur;
     cur = kNoChild;
-    if (leafHits<float>(sc, code, o, d, tmin, &closest, &best, self, excl, anyhit)) {
+    if (leafHits<float>(sc, code, q.o, q.d, q.tmin, &closest, &best, q.self, q.excl, q.anyhit)) {
       found = true;
       return true;
     }
     return false;
   }
-  ZT_D HitRec<float> finish(const SceneView<float>& sc) {
-    if (!found) planeHits<float>(sc, o, d, tmin, &closest, &best, self, excl, anyhit);
+  ZT_D HitRec<float> finish(const SceneView<float>& sc, const Query4& q) {
+    if (!found) planeHits<float>(sc, q.o, q.d, q.tmin, &closest, &best, q.self, q.excl, q.anyhit);
     return HitRec<float>{closest, best};
   }
 };
@@ -477,19 +479,20 @@ struct Trav4 {
 // leaf, so a wave runs node work and leaf work in separate passes.
 ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
                              float tmax, int self, int excl, bool anyhit, int* overflow) {
+  const Query4 q{o, d, tmin, self, excl, anyhit};
   Trav4 t;
-  t.start(o, d, tmin, tmax, self, excl, anyhit);
+  t.start(q, tmax);
   for (;;) {
     bool empty = false;
     while (!t.atLeaf()) {
-      if (t.inner(sc, stk, overflow)) {
+      if (t.inner(sc, stk, q, overflow)) {
         empty = true;
         break;
       }
     }
-    if (empty || t.leaf(sc)) break;
+    if (empty || t.leaf(sc, q)) break;
   }
-  return t.finish(sc);
+  return t.finish(sc, q);
 }
 
 // One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force

@@ -205,8 +205,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
           continue;  // kFinish at once when max_depth == 0
         }
         sh = phase == kShadow;
-        tr.start(st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, sh ? st.sh_tmax : INFINITY, st.self,
-                 sh ? st.sh_excl : -1, sh);
+        tr.start(Query4{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh},
+                 sh ? st.sh_tmax : INFINITY);
         tracing = true;
         break;
       }
@@ -221,6 +221,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     // walk interior nodes, otherwise a node pass (one pop or node each), so
     // no pass pays for the other's code.
     for (;;) {
+      const Query4 q{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh};
       const bool at_leaf = tracing && tr.atLeaf();
       const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
       const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
@@ -229,12 +230,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       if (waiting >= refill) break;
       bool ended = false;
       if (n_leaf >= n_node) {
-        if (at_leaf) ended = tr.leaf(sc);
+        if (at_leaf) ended = tr.leaf(sc, q);
       } else if (tracing && !at_leaf) {
-        ended = tr.inner(sc, stk, &overflow);
+        ended = tr.inner(sc, stk, q, &overflow);
       }
       if (ended) {
-        hit = tr.finish(sc);
+        hit = tr.finish(sc, q);
         tracing = false;
         phase = kResolve;
       }
