@@ -70,15 +70,16 @@ typedef struct zt_gpu_tuning {
   uint32_t tile_h;      /* megakernel: tile height in pixels */
   uint32_t block_size;    /* wavefront: threads per workgroup (0 = 256) */
   uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
-  uint32_t variant;       /* megakernel build: 0 default (fp32: 4, fp64: 1), 1 = registers unconstrained,
-                             2/3/4 = compiled for >= 4/5/6 waves per SIMD */
-  uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64
-                             (0 = megakernel fp32: 8, fp64: 16; wavefront: 16) */
+  uint32_t variant;       /* megakernel build: 1 = registers unconstrained, 2/3/4 = compiled for >= 4/5/6 waves
+                             per SIMD; 0 = default (wave-scheduled: 2; one pixel per lane: fp32 4, fp64 1) */
+  uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64 (0 = wave-scheduled: 12;
+                             one pixel per lane: fp32 8, fp64 16; wavefront: 16) */
   uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
                              nodes (the fp64 path always walks the two-child nodes) */
-  uint32_t schedule;      /* megakernel: 0 or 1 = one pixel per lane, every lane's ray finished before any lane
-                             shades; 2 (fp32 over four-wide nodes) = wave-scheduled: lanes shade and take new
-                             rays or pixels while other lanes keep traversing */
+  uint32_t schedule;      /* megakernel: 1 = one pixel per lane, every lane's ray finished before any lane shades;
+                             2 = wave-scheduled (fp32 over four-wide nodes): wave-uniform node / leaf passes,
+                             lanes shade and take new rays or pixels while other lanes keep traversing;
+                             0 = 2 where it applies, else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
                              1..64 (0 = 32) */
   uint32_t _reserved;

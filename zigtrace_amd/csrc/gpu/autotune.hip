@@ -70,18 +70,20 @@ static std::vector<zt_gpu_tuning> candidates(uint32_t method, uint32_t precision
           tu.tile_h = t[1];
           c.push_back(tu);
         }
+    for (auto& tu : c) tu.schedule = 1;  // the forms above: one pixel per lane
     if (precision == ZT_PRECISION_FP32)  // wave-scheduled forms (four-wide scenes)
-      for (uint32_t variant : {2u, 3u})
-        for (uint32_t refill : {16u, 32u, 48u}) {
-          zt_gpu_tuning tu{};
-          tu.method = method;
-          tu.precision = precision;
-          tu.variant = variant;
-          tu.stack_entries = 8;
-          tu.schedule = 2;
-          tu.refill = refill;
-          c.push_back(tu);
-        }
+      for (uint32_t variant : {1u, 2u})
+        for (uint32_t refill : {24u, 32u, 40u})
+          for (uint32_t cap : {8u, 12u, 16u}) {
+            zt_gpu_tuning tu{};
+            tu.method = method;
+            tu.precision = precision;
+            tu.variant = variant;
+            tu.stack_entries = cap;
+            tu.schedule = 2;
+            tu.refill = refill;
+            c.push_back(tu);
+          }
   } else {
     std::vector<uint32_t> pools;
     for (uint64_t q : {1ull << 20, 1ull << 21, 1ull << 22}) {

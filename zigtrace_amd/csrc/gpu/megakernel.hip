@@ -435,22 +435,27 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);  // render.zig:157,162
   applyBvhWidth(view, tu);
   Params pr = makeParams(s, o);
-  // Defaults measured on the 1080p mesh field (1.3M triangles): 8x8 tiles
-  // (one wave each; many tiles per resident block balance the tail); fp32
-  // compiled for 6 waves per SIMD with 8 LDS stack entries per lane, fp64
-  // unconstrained with 16 entries.
+  // One pixel per lane. Defaults measured on the 1080p mesh field (1.3M
+  // triangles): 8x8 tiles (one wave each; many tiles per resident block
+  // balance the tail); fp32 compiled for 6 waves per SIMD with 8 LDS stack
+  // entries per lane, fp64 unconstrained with 16 entries.
   constexpr bool kF64 = K<R>::exact;
-  const int cap = tu.stack_entries ? (int)tu.stack_entries : (kF64 ? 16 : 8);
-  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
   if (tu.schedule > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "schedule must be 0..2");
   int* err = ctx->errflag.as<int>();
-  const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
   if constexpr (!kF64) {
-    if (view.nodes4 && tu.schedule == 2) {
-      launchWave(ctx, view, pr, dout, tu, variant, cap, err);
+    // fp32 over the four-wide tree: wave-scheduled unless asked otherwise.
+    // Defaults measured on the 1080p mesh field: a build for >= 4 waves per
+    // SIMD (128 VGPRs, no spills), 12 LDS stack entries, refill 32.
+    if (view.nodes4 && tu.schedule != 1) {
+      const int cap = tu.stack_entries ? (int)tu.stack_entries : 12;
+      if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+      launchWave(ctx, view, pr, dout, tu, tu.variant ? tu.variant : 2u, cap, err);
       return;
     }
   }
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : (kF64 ? 16 : 8);
+  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  const uint32_t variant = tu.variant ? tu.variant : (kF64 ? 1u : 4u);
   uint32_t tw = tu.tile_w ? tu.tile_w : 8;
   uint32_t th = tu.tile_h ? tu.tile_h : 8;
   const uint32_t threads = tw * th;
