@@ -17,7 +17,8 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
     12 B/pixel framebuffer write) over the launch's HIP-event duration on the
     context stream the kernel runs on,
   * batch: 256 seeded scenes of uneven size, resolution, spp and depth in one
-    persistent wavefront pass vs one megakernel launch per scene,
+    persistent launch (and in the wavefront renderer's shared path pool) vs
+    one megakernel launch per scene,
   * adjoint: forward + path-replay backward iterations/s of Adam inverse
     rendering on the 256x256, 32 spp differentiable box,
   * the CPU baseline: the oracle (C++ f64 restatement of the reference CPU
@@ -118,8 +119,8 @@ def copy_probe(nbytes: int) -> int:
 
 
 def bench_batch(zt, N, W, steps: int):
-    """256 scenes: one persistent wavefront pass vs one megakernel launch per scene."""
-    import numpy as np
+    """256 scenes: one persistent launch (wave-scheduled megakernel over every scene's tiles;
+    also the wavefront renderer's shared path pool) vs one megakernel launch per scene."""
     t0 = time.time()
     wls = W.batch_scenes()
     log(f"[bench] batch: {len(wls)} scenes built in {time.time() - t0:.1f}s")
@@ -128,27 +129,26 @@ def bench_batch(zt, N, W, steps: int):
     opts = [w.options for w in wls]
     paths = sum(o.width * o.height * o.spp for o in opts)
     outs = [ctx.alloc(o.width * o.height * 12) for o in opts]
-    tu_w = zt.Tuning(method=N.METHOD_WAVEFRONT)
+
+    def timed(fn):
+        fn()  # warm-up
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t) / steps
+
     tu_m = zt.Tuning(method=N.METHOD_MEGAKERNEL)
-    ctx.render_batch_device(ids, opts, tu_w, outs)  # warm-up
-    t = time.perf_counter()
-    for _ in range(steps):
-        ctx.render_batch_device(ids, opts, tu_w, outs)
-    wave_s = (time.perf_counter() - t) / steps
-    for sid, o, d in zip(ids, opts, outs):  # warm-up
-        ctx.render_device(sid, o, tu_m, d)
-    t = time.perf_counter()
-    for _ in range(steps):
-        for sid, o, d in zip(ids, opts, outs):
-            ctx.render_device(sid, o, tu_m, d)
-    mega_s = (time.perf_counter() - t) / steps
+    persistent_s = timed(lambda: ctx.render_batch_device(ids, opts, zt.Tuning(), outs))
+    wavefront_s = timed(lambda: ctx.render_batch_device(ids, opts, zt.Tuning(method=N.METHOD_WAVEFRONT), outs))
+    per_scene_s = timed(lambda: [ctx.render_device(sid, o, tu_m, d) for sid, o, d in zip(ids, opts, outs)])
     for d in outs:
         ctx.free(d)
     ctx.close()
     return {"scenes": len(wls), "paths": paths, "steps": steps,
-            "persistent_wavefront_mpaths_s": round(paths / wave_s / 1e6, 2),
-            "per_scene_megakernel_mpaths_s": round(paths / mega_s / 1e6, 2),
-            "persistent_over_per_scene": round(mega_s / wave_s, 3),
+            "persistent_megakernel_mpaths_s": round(paths / persistent_s / 1e6, 2),
+            "persistent_wavefront_mpaths_s": round(paths / wavefront_s / 1e6, 2),
+            "per_scene_megakernel_mpaths_s": round(paths / per_scene_s / 1e6, 2),
+            "persistent_over_per_scene": round(per_scene_s / persistent_s, 3),
             "prims_total": int(sum(w.scene.info()["num_shapes"] for w in wls))}
 
 

@@ -128,11 +128,14 @@ int zt_gpu_scene_release(zt_ctx* ctx, uint32_t scene_id);
 int zt_gpu_render(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
                   void* out, int out_on_device);
 
-/* Many renders in one persistent wavefront pass (no reference counterpart:
- * the reference renders one scene per call). Scene i is rendered with
- * opts[i] into outs[i] exactly as zt_gpu_render would (same bits); a shared
- * pool of path slots moves across scenes as pixels complete, so scenes of
- * uneven size, spp and depth share the GPU without per-scene launches. */
+/* Many renders in one persistent pass (no reference counterpart: the
+ * reference renders one scene per call). Scene i is rendered with opts[i]
+ * into outs[i] exactly as zt_gpu_render would (same bits). fp32 scenes that
+ * all have four-wide BVHs go through one wave-scheduled megakernel launch
+ * over the tiles of every scene (unless tuning asks for ZT_METHOD_WAVEFRONT
+ * or schedule 1); otherwise the wavefront renderer's pool of path slots
+ * moves across scenes as pixels complete. Either way scenes of uneven size,
+ * spp and depth share the GPU without per-scene launches. */
 int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, const zt_render_options* opts,
                         const zt_gpu_tuning* tuning, void* const* outs, int out_on_device);
 

@@ -1,5 +1,7 @@
 """The batched renderer: many scenes of uneven size, spp and depth in one
-persistent wavefront pass give each scene's single-render image bit for bit."""
+persistent pass (the wave-scheduled megakernel over every scene's tiles, or
+the wavefront renderer's shared path pool) give each scene's single-render
+image bit for bit."""
 import numpy as np
 import pytest
 
@@ -38,4 +40,41 @@ def test_batch_fp64_and_repeated_scene(gpu_ctx):
         single = gpu_ctx.render(sid, o, zt.Tuning(method=N.METHOD_MEGAKERNEL, precision=N.PRECISION_FP64))
         assert np.array_equal(single, img)
     for sid in ids:
+        gpu_ctx.release(sid)
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(), dict(refill=8, tile_w=16, tile_h=8), dict(variant=1, stack_entries=8), dict(method=N.METHOD_MEGAKERNEL),
+])
+def test_wave_megakernel_batch_equals_individual_renders(gpu_ctx, tuning):
+    ws = W.batch_scenes(count=12, seed=9, min_prims=50, max_prims=20000, resolutions=(8, 16, 40, 64),
+                        spps=(1, 3, 8), depth_range=(0, 12))
+    ids = [gpu_ctx.upload(w.scene) for w in ws]
+    opts = [w.options for w in ws]
+    outs = gpu_ctx.render_batch(ids, opts, zt.Tuning(**tuning))
+    for sid, o, img in zip(ids, opts, outs):
+        single = gpu_ctx.render(sid, o, zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=1))
+        assert np.array_equal(single, img), o
+    for sid in ids:
+        gpu_ctx.release(sid)
+
+
+def test_batch_with_one_scene_and_mixed_layouts(gpu_ctx):
+    """One scene alone; and a batch where one scene has no four-wide tree (rendered
+    through the wavefront path as a whole) still matches single renders."""
+    ws = W.batch_scenes(count=3, seed=21, min_prims=30, max_prims=3000, resolutions=(24,), spps=(2,),
+                        depth_range=(3, 6))
+    ids = [gpu_ctx.upload(w.scene) for w in ws]
+    opts = [w.options for w in ws]
+    one = gpu_ctx.render_batch(ids[:1], opts[:1])
+    assert np.array_equal(one[0], gpu_ctx.render(ids[0], opts[0], zt.Tuning(method=N.METHOD_MEGAKERNEL)))
+    u = zt.Scene()
+    m = u.add_material(zt.lambertian((0.5, 0.5, 0.5)))
+    u.add_sphere((0, 0, -3), 1, m)  # not built: brute force, no four-wide tree
+    uid = gpu_ctx.upload(u)
+    uo = zt.Options(16, 16, 2, 4)
+    outs = gpu_ctx.render_batch(ids + [uid], opts + [uo])
+    for sid, o, img in zip(ids + [uid], opts + [uo], outs):
+        assert np.array_equal(img, gpu_ctx.render(sid, o, zt.Tuning(method=N.METHOD_MEGAKERNEL)))
+    for sid in ids + [uid]:
         gpu_ctx.release(sid)

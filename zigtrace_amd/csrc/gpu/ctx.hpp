@@ -131,6 +131,7 @@ struct zt_ctx {
   ztg::DevBuf out, errflag, scratch;
   ztg::DevBuf spill;  // traversal stack overflow, sized per launch (grow-only)
   ztg::DevBuf bands;  // per-XCD tile counters of the persistent megakernel
+  ztg::DevBuf jobs;   // scene table of a many-scene wave-scheduled launch
   ztg::WfBuffers wf;
   std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };

@@ -4,6 +4,8 @@
 #include "../device/dev_path.hpp"
 #include "ctx.hpp"
 
+#include <vector>
+
 namespace ztg {
 
 ztd::Params makeParams(const DevScene& s, const zt_render_options& o);
@@ -12,6 +14,19 @@ void resetErr(zt_ctx* ctx);
 void checkErr(zt_ctx* ctx);
 void uploadMaterialsAndLights(DevScene& s, hipStream_t stream);
 void rebuildLights(DevScene& s, hipStream_t stream);
+
+// One scene of a wave-scheduled launch (fp32, four-wide BVH): its view,
+// render constants, output and its first tile in the launch's tile space.
+struct WaveJob {
+  ztd::SceneView<float> view;
+  ztd::Params pr;
+  float* out;
+  uint32_t tiles_x, first_tile;
+};
+
+// Wave-scheduled megakernel over one or many scenes in one launch (tiles
+// numbered scene after scene). Fills tiles_x / first_tile.
+void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning& tu);
 
 // Blocks of `kernel` the device can hold at once (occupancy x CUs).
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds);

@@ -119,10 +119,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
 // their traversal state. Each pixel's computation is the same sequence of
 // device functions as renderPixel's, so the image is identical; only the
 // interleaving of lanes changes.
-template <int W>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
-    SceneView<float> sc, Params pr, float* __restrict__ out, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
-    uint32_t ntiles, int cap, uint32_t refill, int* spill, uint32_t* band_next, int* err) {
+// The scenes a wave-scheduled launch renders, with their tiles numbered in
+// one index space. One scene: everything in kernel arguments. Many scenes:
+// a table in device memory, each lane reading its own pixel's scene.
+struct OneScene {
+  SceneView<float> sc;
+  Params pr;
+  float* out;
+  uint32_t tiles_x;
+  ZT_D uint32_t sceneOf(uint32_t) const { return 0; }
+  ZT_D uint32_t firstTile(uint32_t) const { return 0; }
+  ZT_D uint32_t tilesX(uint32_t) const { return tiles_x; }
+  ZT_D const SceneView<float>& view(uint32_t) const { return sc; }
+  ZT_D const Params& params(uint32_t) const { return pr; }
+  ZT_D float* output(uint32_t) const { return out; }
+};
+
+struct ManyScenes {
+  const WaveJob* jobs;
+  uint32_t n;
+  ZT_D uint32_t sceneOf(uint32_t tile) const {  // last job whose first tile <= tile
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].first_tile <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+  ZT_D uint32_t firstTile(uint32_t si) const { return jobs[si].first_tile; }
+  ZT_D uint32_t tilesX(uint32_t si) const { return jobs[si].tiles_x; }
+  ZT_D const SceneView<float>& view(uint32_t si) const { return jobs[si].view; }
+  ZT_D const Params& params(uint32_t si) const { return jobs[si].pr; }
+  ZT_D float* output(uint32_t si) const { return jobs[si].out; }
+};
+
+template <typename Src>
+ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap,
+                             uint32_t refill, int* spill, uint32_t* band_next, int* err) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, cap, spill);
@@ -130,11 +164,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
   const uint32_t tile_px = tile_w * tile_h;
   const uint32_t home = blockIdx.x % 8;
   // Wave-uniform tile cursor.
-  uint32_t band_k = 0, tile = 0, next_px = tile_px;
+  uint32_t band_k = 0, tile = 0, tile_scene = 0, next_px = tile_px;
   bool tiles_done = false;
   // Lane state.
   bool has = false, tracing = false, sh = false;
-  uint32_t x = 0, y = 0, sample = 0;
+  uint32_t si = 0, x = 0, y = 0, sample = 0;
   int phase = kCamera;
   Rng rng{};
   V3<float> sum = splat(0.0f);
@@ -159,6 +193,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
           t = __shfl(t, 0);
           if (t < bandStart(band + 1, ntiles)) {
             tile = t;
+            tile_scene = src.sceneOf(t);
             next_px = 0;
             break;
           }
@@ -169,13 +204,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       const uint32_t idx = next_px + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
       next_px = min(tile_px, next_px + (uint32_t)__popcll(need));
       if (!has && idx < tile_px) {
-        const uint32_t px = (tile % tiles_x) * tile_w + idx % tile_w;
-        const uint32_t py = (tile / tiles_x) * tile_h + idx / tile_w;
-        if (px < pr.width && py < pr.height) {
+        const Params& tp = src.params(tile_scene);
+        const uint32_t lt = tile - src.firstTile(tile_scene), tx = src.tilesX(tile_scene);
+        const uint32_t px = (lt % tx) * tile_w + idx % tile_w;
+        const uint32_t py = (lt / tx) * tile_h + idx / tile_w;
+        if (px < tp.width && py < tp.height) {
+          si = tile_scene;
           x = px;
           y = py;
           has = true;
-          rng = rngForStream(pr.seed, (uint64_t)py * pr.width + px);  // render.zig:135
+          rng = rngForStream(tp.seed, (uint64_t)py * tp.width + px);  // render.zig:135
           sum = splat(0.0f);
           sample = 0;
           phase = kCamera;
@@ -184,6 +222,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     }
     // Lanes with no ray in flight advance their path to its next ray.
     if (has && !tracing) {
+      const SceneView<float>& sc = src.view(si);
+      const Params& pr = src.params(si);
       for (;;) {
         if (phase == kResolve) phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc, pr, hit, rng, st);
         if (phase == kFinish) {
@@ -193,6 +233,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
         }
         if (phase == kCamera) {
           if (sample == pr.spp) {
+            float* out = src.output(si);
             const uint64_t pix = (uint64_t)y * pr.width + x;
             const float inv_spp = 1.0f / (float)pr.spp;
             out[3 * pix + 0] = sum.x * inv_spp;
@@ -230,18 +271,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       if (waiting >= refill) break;
       bool ended = false;
       if (n_leaf >= n_node) {
-        if (at_leaf) ended = tr.leaf(sc, q);
+        if (at_leaf) ended = tr.leaf(src.view(si), q);
       } else if (tracing && !at_leaf) {
-        ended = tr.inner(sc, stk, q, &overflow);
+        ended = tr.inner(src.view(si), stk, q, &overflow);
       }
       if (ended) {
-        hit = tr.finish(sc, q);
+        hit = tr.finish(src.view(si), q);
         tracing = false;
         phase = kResolve;
       }
     }
   }
   if (overflow) atomicOr(err, 1);
+}
+
+template <int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
+    OneScene src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
+    uint32_t* band_next, int* err) {
+  megakernelWaveBody(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
+}
+
+// Many scenes in one persistent launch (zt_gpu_render_batch): the same
+// kernel over the concatenated tiles of every scene.
+template <int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWaveBatch(
+    ManyScenes src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
+    uint32_t* band_next, int* err) {
+  megakernelWaveBody(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
 }
 
 __global__ void clearBands(uint32_t* band_next) {
@@ -370,9 +427,9 @@ void checkOptions(const zt_render_options* o) {
     throw Error(ZT_ERR_INVALID_ARGUMENT, "image size or spp beyond the device limits");
 }
 
-// Wave-scheduled fp32 render over the four-wide BVH (see megakernelWave).
-static void launchWave(zt_ctx* ctx, const SceneView<float>& view, const Params& pr, float* dout,
-                       const zt_gpu_tuning& tu, uint32_t variant, int cap, int* err) {
+// Wave-scheduled fp32 render over the four-wide BVH (see megakernelWaveBody)
+// of one scene (kernel arguments) or many (a scene table in device memory).
+void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning& tu) {
   // A tile is a wave's unit of work: 8x8 gives every lane one pixel to start
   // with and keeps enough tiles for every resident wave on small images.
   const uint32_t tw = tu.tile_w ? tu.tile_w : 8;
@@ -381,41 +438,53 @@ static void launchWave(zt_ctx* ctx, const SceneView<float>& view, const Params& 
     throw Error(ZT_ERR_INVALID_ARGUMENT, "wave-scheduled tiles must hold 64..4096 pixels");
   const uint32_t refill = tu.refill ? tu.refill : 32;
   if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
-  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
-  const uint32_t ntiles = tiles_x * ((pr.height + th - 1) / th);
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : 12;
+  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  const uint32_t variant = tu.variant ? tu.variant : 2u;
+  if (variant < 1 || variant > 4) throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+  uint64_t ntiles = 0;
+  for (WaveJob& j : jobs) {
+    j.tiles_x = (j.pr.width + tw - 1) / tw;
+    j.first_tile = (uint32_t)ntiles;
+    ntiles += (uint64_t)j.tiles_x * ((j.pr.height + th - 1) / th);
+  }
+  if (ntiles == 0) return;
+  if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
+  const bool many = jobs.size() > 1;
   const uint32_t threads = 64;
   const size_t lds = stackBytes(cap, (int)threads);
-  const void* fn = nullptr;
-  switch (variant) {
-    case 1: fn = (const void*)megakernelWave<1>; break;
-    case 2: fn = (const void*)megakernelWave<4>; break;
-    case 3: fn = (const void*)megakernelWave<5>; break;
-    case 4: fn = (const void*)megakernelWave<6>; break;
-    default: throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
-  }
-  const uint32_t grid = std::min<uint32_t>(ntiles, residentBlocks(fn, threads, lds));
+  static const void* const kOne[4] = {(const void*)megakernelWave<1>, (const void*)megakernelWave<4>,
+                                      (const void*)megakernelWave<5>, (const void*)megakernelWave<6>};
+  static const void* const kMany[4] = {(const void*)megakernelWaveBatch<1>, (const void*)megakernelWaveBatch<4>,
+                                       (const void*)megakernelWaveBatch<5>, (const void*)megakernelWaveBatch<6>};
+  const void* fn = (many ? kMany : kOne)[variant - 1];
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
   ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
   ctx->bands.ensure(8 * sizeof(uint32_t));
   uint32_t* bands = ctx->bands.as<uint32_t>();
   int* spill = ctx->spill.as<int>();
+  int* err = ctx->errflag.as<int>();
+  const uint32_t nt = (uint32_t)ntiles;
   hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
-  switch (variant) {
-    case 1:
-      hipLaunchKernelGGL(megakernelWave<1>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, refill, spill, bands, err);
-      break;
-    case 2:
-      hipLaunchKernelGGL(megakernelWave<4>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, refill, spill, bands, err);
-      break;
-    case 3:
-      hipLaunchKernelGGL(megakernelWave<5>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, refill, spill, bands, err);
-      break;
-    default:
-      hipLaunchKernelGGL(megakernelWave<6>, dim3(grid), dim3(threads), lds, ctx->stream, view, pr, dout, tw, th,
-                         tiles_x, ntiles, cap, refill, spill, bands, err);
-      break;
+  if (!many) {
+    const OneScene src{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
+    switch (variant) {
+      case 1: hipLaunchKernelGGL(megakernelWave<1>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      case 2: hipLaunchKernelGGL(megakernelWave<4>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      case 3: hipLaunchKernelGGL(megakernelWave<5>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      default: hipLaunchKernelGGL(megakernelWave<6>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+    }
+  } else {
+    ctx->jobs.ensure(jobs.size() * sizeof(WaveJob));
+    ZT_HIP(hipMemcpyAsync(ctx->jobs.p, jobs.data(), jobs.size() * sizeof(WaveJob), hipMemcpyHostToDevice,
+                          ctx->stream));
+    const ManyScenes src{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
+    switch (variant) {
+      case 1: hipLaunchKernelGGL(megakernelWaveBatch<1>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      case 2: hipLaunchKernelGGL(megakernelWaveBatch<4>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      case 3: hipLaunchKernelGGL(megakernelWaveBatch<5>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+      default: hipLaunchKernelGGL(megakernelWaveBatch<6>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
+    }
   }
   ZT_HIP(hipGetLastError());
   ctx->stats.launches += 2;
@@ -447,9 +516,8 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
     // Defaults measured on the 1080p mesh field: a build for >= 4 waves per
     // SIMD (128 VGPRs, no spills), 12 LDS stack entries, refill 32.
     if (view.nodes4 && tu.schedule != 1) {
-      const int cap = tu.stack_entries ? (int)tu.stack_entries : 12;
-      if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
-      launchWave(ctx, view, pr, dout, tu, tu.variant ? tu.variant : 2u, cap, err);
+      std::vector<WaveJob> jobs{WaveJob{view, pr, dout, 0, 0}};
+      launchWaveJobs(ctx, jobs, tu);
       return;
     }
   }

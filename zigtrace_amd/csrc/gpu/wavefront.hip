@@ -412,6 +412,22 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
         first += (uint64_t)opts[i].width * opts[i].height;
         paths += (uint64_t)opts[i].width * opts[i].height * opts[i].spp;
       }
+      // fp32 scenes that all have four-wide trees go through the persistent
+      // wave-scheduled megakernel (one launch over every scene's tiles)
+      // unless the wavefront renderer or the one-pixel-per-lane schedule is
+      // asked for; everything else through the wavefront renderer.
+      bool wave = !K<R>::exact && tu.method != ZT_METHOD_WAVEFRONT && tu.schedule != 1;
+      for (uint32_t i = 0; wave && i < n; ++i) wave = scenes[i].view.nodes4 != nullptr;
+      if constexpr (!K<R>::exact) {
+        if (wave) {
+          std::vector<WaveJob> jobs(n);
+          for (uint32_t i = 0; i < n; ++i) jobs[i] = WaveJob{scenes[i].view, scenes[i].pr, scenes[i].out, 0, 0};
+          resetErr(ctx);
+          launchWaveJobs(ctx, jobs, tu);
+          checkErr(ctx);
+          return;
+        }
+      }
       runWavefront<R>(ctx, scenes, tu);
     };
     if (f64) build(double{});
