@@ -54,7 +54,7 @@ HIT = np.dtype([("t", "<f8"), ("point", "<f8", (3,)), ("normal", "<f8", (3,)), (
                 ("material", "<u4"), ("hit", "<i4"), ("prim", "<i4")])
 TUNING = np.dtype([("method", "<u4"), ("precision", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"),
                    ("block_size", "<u4"), ("queue_paths", "<u4"), ("variant", "<u4"), ("stack_entries", "<u4"),
-                   ("bvh_width", "<u4"), ("schedule", "<u4"), ("refill", "<u4"), ("_reserved", "<u4")])
+                   ("bvh_width", "<u4"), ("schedule", "<u4"), ("refill", "<u4"), ("node_bytes", "<u4")])
 STATS = np.dtype([("kernel_ms", "<f8"), ("upload_ms", "<f8"), ("launches", "<u8"), ("paths", "<u8"),
                   ("extend_rays", "<u8"), ("shadow_rays", "<u8")])
 SCENE_INFO = np.dtype([("num_materials", "<u8"), ("num_shapes", "<u8"), ("num_planes", "<u8"), ("num_nodes", "<u8"),

@@ -161,6 +161,21 @@ struct __align__(128) DNode4 {
   int4 _pad;
 };
 static_assert(sizeof(DNode4) == 128, "one four-wide node per 128-byte line");
+
+// The same four-wide node in 64 bytes: child boxes as 8-bit offsets from a
+// per-node origin in steps of a power of two per axis. A bound decodes as
+// fma(q, 2^e, origin) -- the product is exact, so one rounding -- and the
+// host picks every q so that the decoded box contains the fp32 box it
+// replaces. qlo / qhi hold byte k for child k; an empty slot has code
+// kNoChild and an inverted box.
+struct __align__(64) DNode4Q {
+  float ox, oy, oz;
+  uint32_t exps;  // bytes 0..2: e_x + 127, e_y + 127, e_z + 127
+  int4 child;
+  uint32_t qlox, qhix, qloy, qhiy, qloz, qhiz;
+  uint32_t _pad[2];
+};
+static_assert(sizeof(DNode4Q) == 64, "one compressed four-wide node per 64 bytes");
 static constexpr uint32_t kMaxLeafCount = 15;
 static constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
 ZT_HD int leafCode(uint32_t first, uint32_t count) { return -1 - (int)((first << 4) | count); }
@@ -222,7 +237,8 @@ struct CamD {
 template <typename R>
 struct SceneView {
   const DNode* nodes;
-  const DNode4* nodes4;  // fp32 only: four-wide BVH (null: two-child traversal)
+  const DNode4* nodes4;    // fp32 only: four-wide BVH (null: two-child traversal)
+  const DNode4Q* nodes4q;  // fp32 only: the four-wide BVH in 64-byte nodes (null: use nodes4)
   const void* prims;  // PrimF for float, PrimD for double
   const Plane<R>* planes;
   const Mat<R>* mats;

@@ -293,6 +293,14 @@ ZT_D HitRec<R> traverse(const SceneView<R>& sc, const LaneStack& stk, V3<R> o, V
   return HitRec<R>{closest, best};
 }
 
+// The four bounds packed in `q` (byte k = child k) decoded as
+// fma(q_k, s, origin): q_k * s is exact (s is a power of two), so the result
+// is the one rounding of the exact bound, as the host assumed.
+ZT_D float4 decodeBounds(uint32_t q, float s, float origin) {
+  return make_float4(__builtin_fmaf((float)(q & 0xFFu), s, origin), __builtin_fmaf((float)((q >> 8) & 0xFFu), s, origin),
+                     __builtin_fmaf((float)((q >> 16) & 0xFFu), s, origin), __builtin_fmaf((float)(q >> 24), s, origin));
+}
+
 // Compare-exchange of (entry distance, child code) pairs: after it, a holds
 // the nearer entry.
 ZT_D void sortChild(float& ka, int& ca, float& kb, int& cb) {
@@ -432,9 +440,39 @@ struct Trav4 {
       if (stk.tval[sp * stk.stride] <= closest) cur = stk.code[sp * stk.stride];
       return false;
     }
-    const DNode4* n = sc.nodes4 + cur;
-    const float4 lx = n->lox, ly = n->loy, lz = n->loz, hx = n->hix, hy = n->hiy, hz = n->hiz;
-    int c0 = n->child.x, c1 = n->child.y, c2 = n->child.z, c3 = n->child.w;
+    float4 lx, ly, lz, hx, hy, hz;
+    int c0, c1, c2, c3;
+    if (sc.nodes4q) {
+      const uint4* p = (const uint4*)(sc.nodes4q + cur);
+      const uint4 a = p[0], b = p[1], c = p[2];
+      const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+      const float sx = __uint_as_float((a.w & 0xFFu) << 23);
+      const float sy = __uint_as_float(((a.w >> 8) & 0xFFu) << 23);
+      const float sz = __uint_as_float(((a.w >> 16) & 0xFFu) << 23);
+      c0 = (int)b.x;
+      c1 = (int)b.y;
+      c2 = (int)b.z;
+      c3 = (int)b.w;
+      lx = decodeBounds(c.x, sx, ox);
+      hx = decodeBounds(c.y, sx, ox);
+      ly = decodeBounds(c.z, sy, oy);
+      hy = decodeBounds(c.w, sy, oy);
+      const uint2 d = *(const uint2*)(p + 3);
+      lz = decodeBounds(d.x, sz, oz);
+      hz = decodeBounds(d.y, sz, oz);
+    } else {
+      const DNode4* n = sc.nodes4 + cur;
+      lx = n->lox;
+      ly = n->loy;
+      lz = n->loz;
+      hx = n->hix;
+      hy = n->hiy;
+      hz = n->hiz;
+      c0 = n->child.x;
+      c1 = n->child.y;
+      c2 = n->child.z;
+      c3 = n->child.w;
+    }
     float k0, k1, k2, k3;
     const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, q.o, inv, nx, ny, nz, q.tmin, closest, &k0);

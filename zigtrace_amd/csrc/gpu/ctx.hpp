@@ -83,7 +83,7 @@ struct LightCandidate : ztd::Light<double> {
 // One uploaded scene: the device layout in both precisions plus what the
 // host needs to rebuild per-render constants.
 struct DevScene {
-  DevBuf nodes, nodes4, primsF, primsD, planesF, planesD, matsF, matsD, lightsF, lightsD, cdf, prim_light;
+  DevBuf nodes, nodes4, nodes4q, primsF, primsD, planesF, planesD, matsF, matsD, lightsF, lightsD, cdf, prim_light;
   int num_nodes = 0, num_nodes4 = 0, num_prims = 0, num_planes = 0, num_lights = 0, num_mats = 0;
   int tree_depth = 0, tree_depth4 = 0;
   bool brute_force = false;  // no BVH handed over (or one too deep for the stack)

@@ -42,7 +42,10 @@ template <typename R>
 void applyBvhWidth(ztd::SceneView<R>& v, const zt_gpu_tuning& tu) {
   if (tu.bvh_width != 0 && tu.bvh_width != 2 && tu.bvh_width != 4)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "bvh_width must be 0, 2 or 4");
+  if (tu.node_bytes != 0 && tu.node_bytes != 64 && tu.node_bytes != 128)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "node_bytes must be 0, 64 or 128");
   if (tu.bvh_width == 2) v.nodes4 = nullptr;
+  if (tu.bvh_width == 2 || tu.node_bytes == 128) v.nodes4q = nullptr;
 }
 
 template <typename R>

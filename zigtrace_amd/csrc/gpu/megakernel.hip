@@ -382,7 +382,8 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
                                                        int mode, zt_hit* out, int* spill, int* err) {
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, kLds, spill);
-  if (mode == 2) sc.nodes4 = nullptr;  // the two-child traversal
+  if (mode == 2) sc.nodes4 = nullptr;   // the two-child traversal
+  if (mode == 2 || mode == 3) sc.nodes4q = nullptr;  // mode 3: four-wide 128-byte nodes
   int overflow = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = intersectOne<R>(sc, stk, rays[i], tmin, tmax, mode, &overflow);
@@ -670,7 +671,7 @@ int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_
   return guarded(ctx, [&] {
     if (n == 0) return;
     if (!rays || !out) throw Error(ZT_ERR_INVALID_ARGUMENT, "null buffer");
-    if (precision > ZT_PRECISION_FP64 || mode < 0 || mode > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad enum");
+    if (precision > ZT_PRECISION_FP64 || mode < 0 || mode > 3) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad enum");
     DevScene& s = sceneOf(ctx, scene_id);
     setDevice(ctx);
     DevBuf drays, dout;

@@ -64,6 +64,7 @@ SceneView<R> viewOf(const DevScene& s) {
   v.nodes = s.brute_force ? nullptr : s.nodes.as<DNode>();
   v.num_nodes = s.brute_force ? 0 : s.num_nodes;
   v.nodes4 = (std::is_same<R, float>::value && !s.brute_force && s.num_nodes4) ? s.nodes4.as<DNode4>() : nullptr;
+  v.nodes4q = (v.nodes4 && s.nodes4q.p) ? s.nodes4q.as<DNode4Q>() : nullptr;
   if constexpr (std::is_same<R, float>::value) {
     v.prims = s.primsF.p;
     v.planes = s.planesF.as<Plane<float>>();
@@ -260,6 +261,94 @@ struct Collapse4 {
     o.child = make_int4(code[0], code[1], code[2], code[3]);
     o._pad = make_int4(0, 0, 0, 0);
     return idx;
+  }
+};
+
+// Compresses four-wide nodes to 64 bytes (DNode4Q). Per node and axis the
+// origin is the smallest child lower bound and the step s = 2^e the smallest
+// power of two that lets every child bound be reached within 255 steps;
+// each lower bound q is the largest with fma(q, s, origin) <= lo and each
+// upper bound the smallest with fma(q, s, origin) >= hi, evaluated with the
+// same single-rounding fma the device decodes with, so a decoded box always
+// contains the fp32 box (and the traversal's hits are those of the fp32
+// boxes, up to exact ties in t).
+struct Quantize4 {
+  static bool axis(const float* lo, const float* hi, const bool* valid, float* origin, int* e_out, uint8_t* qlo,
+                   uint8_t* qhi) {
+    float o = INFINITY, m = -INFINITY;
+    for (int k = 0; k < 4; ++k)
+      if (valid[k]) {
+        o = std::min(o, lo[k]);
+        m = std::max(m, hi[k]);
+      }
+    if (!(o <= m)) {  // no valid child: any inverted box
+      *origin = 0;
+      *e_out = 0;
+      for (int k = 0; k < 4; ++k) qlo[k] = 255, qhi[k] = 0;
+      return true;
+    }
+    const double ext = (double)m - (double)o;
+    int e = ext > 0 ? (int)std::ceil(std::log2(ext / 255.0)) - 1 : -126;
+    e = std::max(-126, std::min(127, e));
+    for (; e <= 127; ++e) {
+      const float st = std::ldexp(1.0f, e);
+      bool ok = true;
+      for (int k = 0; k < 4 && ok; ++k) {
+        if (!valid[k]) {
+          qlo[k] = 255;
+          qhi[k] = 0;
+          continue;
+        }
+        double fl = std::floor(((double)lo[k] - (double)o) / st);
+        int ql = (int)std::max(0.0, std::min(255.0, fl));
+        while (ql > 0 && std::fmaf((float)ql, st, o) > lo[k]) --ql;
+        while (ql < 255 && std::fmaf((float)(ql + 1), st, o) <= lo[k]) ++ql;
+        if (std::fmaf((float)ql, st, o) > lo[k]) ok = false;
+        double ch = std::ceil(((double)hi[k] - (double)o) / st);
+        int qh = (int)std::max(0.0, std::min(256.0, ch));
+        while (qh <= 255 && std::fmaf((float)qh, st, o) < hi[k]) ++qh;
+        while (qh > 0 && qh <= 255 && std::fmaf((float)(qh - 1), st, o) >= hi[k]) --qh;
+        if (qh > 255) ok = false;
+        qlo[k] = (uint8_t)ql;
+        qhi[k] = (uint8_t)std::min(qh, 255);
+      }
+      if (ok) {
+        *origin = o;
+        *e_out = e;
+        return true;
+      }
+    }
+    return false;
+  }
+  static bool node(const DNode4& n, DNode4Q* out) {
+    const int code[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+    bool valid[4];
+    for (int k = 0; k < 4; ++k) valid[k] = code[k] != kNoChild;
+    const float lo[3][4] = {{n.lox.x, n.lox.y, n.lox.z, n.lox.w}, {n.loy.x, n.loy.y, n.loy.z, n.loy.w},
+                            {n.loz.x, n.loz.y, n.loz.z, n.loz.w}};
+    const float hi[3][4] = {{n.hix.x, n.hix.y, n.hix.z, n.hix.w}, {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w},
+                            {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w}};
+    float origin[3];
+    int e[3];
+    uint8_t ql[3][4], qh[3][4];
+    for (int a = 0; a < 3; ++a)
+      if (!axis(lo[a], hi[a], valid, &origin[a], &e[a], ql[a], qh[a])) return false;
+    auto pack = [](const uint8_t* q) {
+      return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+    };
+    *out = DNode4Q{};
+    out->ox = origin[0];
+    out->oy = origin[1];
+    out->oz = origin[2];
+    out->exps = (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) | ((uint32_t)(e[2] + 127) << 16);
+    out->child = n.child;
+    out->qlox = pack(ql[0]);
+    out->qhix = pack(qh[0]);
+    out->qloy = pack(ql[1]);
+    out->qhiy = pack(qh[1]);
+    out->qloz = pack(ql[2]);
+    out->qhiz = pack(qh[2]);
+    return true;
   }
 };
 
@@ -469,6 +558,10 @@ int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id)
         if (3 * c4.max_depth + 1 <= 8 + kSpill) {
           s->num_nodes4 = (int)c4.out.size();
           s->nodes4.upload(c4.out, ctx->stream);
+          std::vector<DNode4Q> q(c4.out.size());
+          bool ok = true;
+          for (size_t i = 0; i < q.size() && ok; ++i) ok = Quantize4::node(c4.out[i], &q[i]);
+          if (ok) s->nodes4q.upload(q, ctx->stream);  // else only the 128-byte form
         }
       }
     } else {

@@ -48,12 +48,13 @@ class Tuning:
     bvh_width: int = 0
     schedule: int = 0
     refill: int = 0
+    node_bytes: int = 0
 
     def record(self) -> np.ndarray:
         return N.record(N.TUNING, method=self.method, precision=self.precision, tile_w=self.tile_w,
                         tile_h=self.tile_h, block_size=self.block_size, queue_paths=self.queue_paths,
                         variant=self.variant, stack_entries=self.stack_entries, bvh_width=self.bvh_width,
-                        schedule=self.schedule, refill=self.refill)
+                        schedule=self.schedule, refill=self.refill, node_bytes=self.node_bytes)
 
 
 def render(scene: Scene, opts: Options, tuning: Tuning | None = None) -> np.ndarray:
@@ -147,7 +148,7 @@ class Context:
         ms = C.c_double()
         self._check(N.lib().zt_gpu_autotune(self._h, scene_id, N.ptr(opts.record()), method, precision, probe_spp,
                                             N.ptr(rec), C.byref(ms)))
-        return Tuning(**{k: int(rec[k][0]) for k in N.TUNING.names if not k.startswith("_")}), ms.value
+        return Tuning(**{k: int(rec[k][0]) for k in N.TUNING.names}), ms.value
 
     def render_batch(self, scene_ids, options, tuning: Tuning | None = None, outs=None):
         """Renders several scenes in one wavefront pass; returns host images."""
