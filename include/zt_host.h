@@ -67,6 +67,14 @@ int zt_scene_parse(const char* src, uint64_t len, zt_scene** out, uint64_t* diag
  * ZT_ERR_TOO_MANY_PRIMITIVES beyond maxInt(u32)/2. */
 int zt_bvh_build(const zt_shape* shapes, uint64_t n, zt_bvh_node* nodes_out, uint64_t* num_nodes,
                  uint32_t* indices_out);
+/* Bvh.build with an explicit worker count (no reference counterpart: the
+ * reference builds on one thread). threads = 1 is the sequential build,
+ * 0 = up to 16 hardware threads; every count gives the same node array and
+ * index permutation bit for bit (subtrees own disjoint index ranges and are
+ * numbered afterwards in the reference's allocation order). zt_bvh_build and
+ * zt_scene_build use 0. */
+int zt_bvh_build_threads(const zt_shape* shapes, uint64_t n, uint32_t threads, zt_bvh_node* nodes_out,
+                         uint64_t* num_nodes, uint32_t* indices_out);
 
 /* render.render (render.zig:152-180) on the GPU: validates options first
  * (ZT_ERR_INVALID_OPTIONS), uploads the scene when it changed, renders and

@@ -43,6 +43,32 @@ def test_random_bvh_bit_identical_to_oracle(n):
     assert nodes.tobytes() == on.tobytes() and idx.tobytes() == oi.tobytes()
 
 
+@pytest.mark.parametrize("n,threads", [(70000, 2), (120000, 8), (200000, 0)])
+def test_parallel_bvh_build_bit_identical(n, threads):
+    """The multi-threaded build numbers nodes as the sequential one appends them:
+    same node array and permutation as threads=1 and as the oracle's build."""
+    rng = np.random.default_rng(n)
+    shapes = random_shapes(rng, n)
+    seq_nodes, seq_idx = zt.bvh_build(shapes, threads=1)
+    par_nodes, par_idx = zt.bvh_build(shapes, threads=threads)
+    assert par_nodes.tobytes() == seq_nodes.tobytes() and par_idx.tobytes() == seq_idx.tobytes()
+    on, oi = O.bvh_build(shapes.view(O.SHAPE))
+    assert seq_nodes.tobytes() == on.tobytes() and seq_idx.tobytes() == oi.tobytes()
+
+
+def test_parallel_bvh_build_on_degenerate_input():
+    """Many coincident centroids (median-split path) and duplicated triangles."""
+    n = 80000
+    shapes = np.zeros(n, dtype=N.SHAPE)
+    shapes["kind"] = N.SPHERE
+    shapes["material"] = 0
+    shapes["v"][:, :3] = np.repeat(np.arange(n // 1000)[:, None] * 0.5, 1000, axis=0)[:, :1]
+    shapes["v"][:, 3] = 0.25
+    a = zt.bvh_build(shapes, threads=1)
+    b = zt.bvh_build(shapes, threads=6)
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
+
+
 def test_coincident_centroids_bvh_bit_identical():  # bvh.zig:356-367: median split path
     shapes = np.zeros(40, dtype=N.SHAPE)
     shapes["kind"] = N.SPHERE

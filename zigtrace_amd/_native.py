@@ -138,6 +138,7 @@ _SIGS = {
     "zt_scene_get_view": (C.c_int, [C.c_void_p, C.c_void_p]),
     "zt_scene_parse": (C.c_int, [C.c_char_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint64]),
     "zt_bvh_build": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "zt_bvh_build_threads": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_default_context": (C.c_int, [C.c_void_p]),
     "zt_scene_device_id": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

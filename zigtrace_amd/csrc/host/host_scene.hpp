@@ -14,7 +14,9 @@ struct HostLights {  // light.zig:59-63
   double total = 0;
 };
 
-int buildBvh(const zt_shape* shapes, uint64_t n, std::vector<zt_bvh_node>* nodes, std::vector<uint32_t>* indices);
+// threads: 1 = the sequential build, 0 = up to 16 hardware threads (same bits).
+int buildBvh(const zt_shape* shapes, uint64_t n, std::vector<zt_bvh_node>* nodes, std::vector<uint32_t>* indices,
+             unsigned threads = 0);
 uint64_t bvhDepth(const std::vector<zt_bvh_node>& nodes);
 void buildLights(const std::vector<zt_shape>& shapes, const std::vector<zt_material>& materials, HostLights* out);
 

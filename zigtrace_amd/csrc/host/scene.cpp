@@ -211,10 +211,15 @@ int zt_scene_get_view(const zt_scene* s, zt_scene_view* v) {
 
 int zt_bvh_build(const zt_shape* shapes, uint64_t n, zt_bvh_node* nodes_out, uint64_t* num_nodes,
                  uint32_t* indices_out) {
+  return zt_bvh_build_threads(shapes, n, 0, nodes_out, num_nodes, indices_out);
+}
+
+int zt_bvh_build_threads(const zt_shape* shapes, uint64_t n, uint32_t threads, zt_bvh_node* nodes_out,
+                         uint64_t* num_nodes, uint32_t* indices_out) {
   if ((n && !shapes) || !num_nodes) return ZT_ERR_INVALID_ARGUMENT;
   std::vector<zt_bvh_node> nodes;
   std::vector<uint32_t> idx;
-  const int rc = zth::buildBvh(shapes, n, &nodes, &idx);
+  const int rc = zth::buildBvh(shapes, n, &nodes, &idx, threads);
   if (rc != ZT_OK) return rc;
   *num_nodes = nodes.size();
   if (nodes_out && !nodes.empty()) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(zt_bvh_node));

@@ -180,12 +180,14 @@ class Scene:
         }
 
 
-def bvh_build(shapes: np.ndarray):
-    """Bvh.build (bvh.zig:41-67) over a shape list: (nodes, indices)."""
+def bvh_build(shapes: np.ndarray, threads: int | None = None):
+    """Bvh.build (bvh.zig:41-67) over a shape list: (nodes, indices). `threads`: 1 = the
+    sequential build, 0 or None = the parallel build on up to 16 threads (same bits)."""
     shapes = np.ascontiguousarray(shapes, dtype=N.SHAPE)
     n = len(shapes)
     nodes = np.zeros(max(2 * n - 1, 1), dtype=N.NODE)
     idx = np.zeros(max(n, 1), dtype=np.uint32)
     nn = C.c_uint64()
-    N.check(N.lib().zt_bvh_build(N.ptr(shapes) if n else None, n, N.ptr(nodes), C.byref(nn), N.ptr(idx)))
+    N.check(N.lib().zt_bvh_build_threads(N.ptr(shapes) if n else None, n, threads or 0, N.ptr(nodes), C.byref(nn),
+                                          N.ptr(idx)))
     return nodes[: nn.value], idx[:n]
