@@ -82,8 +82,8 @@ typedef struct zt_gpu_tuning {
                              0 = 2 where it applies, else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
                              1..64 (0 = 32) */
-  uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 64 = 64-byte nodes (8-bit child boxes, decoded to
-                             boxes that contain the fp32 boxes), 128 = the fp32 boxes themselves */
+  uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
+                             (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */
@@ -159,10 +159,10 @@ int zt_gpu_trace_rays(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64
                       uint64_t max_depth, uint32_t light_sampling, uint32_t precision, double* out_rgb);
 
 /* Scene.intersect (scene.zig:108-116) for n rays: mode 0 = BVH + planes as
- * the renderer traverses it by default (fp32: the four-wide BVH in 64-byte
+ * the renderer traverses it by default (fp32: the four-wide BVH in 128-byte
  * nodes when the scene has one), mode 1 = brute force over every primitive
  * (geometry.zig:129-139), mode 2 = the two-child BVH + planes, mode 3 = the
- * four-wide BVH in 128-byte nodes + planes. */
+ * four-wide BVH in 64-byte nodes + planes. */
 int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_t n, double t_min, double t_max,
                      int mode, uint32_t precision, zt_hit* out);
 

@@ -48,7 +48,7 @@ def test_fp32_uniform_is_truncated_f64_uniform(gpu_ctx):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 50, 400, 5000])
 @pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
-@pytest.mark.parametrize("mode", [0, 2, 3])  # default (fp32: four-wide, 64-byte nodes); two-child; 128-byte
+@pytest.mark.parametrize("mode", [0, 2, 3])  # default (fp32: four-wide, 128-byte nodes); two-child; 64-byte
 def test_bvh_traversal_equals_brute_force(gpu_ctx, n, precision, mode):
     rng = np.random.default_rng(2026 + n)
     sc = random_scene(rng, n)

@@ -101,12 +101,12 @@ def test_two_child_and_four_wide_traversal_agree():
     s, o = wl.scene, wl.options
     four = zt.render(s, o, zt.Tuning(method=MEGA))
     two = zt.render(s, o, zt.Tuning(method=MEGA, bvh_width=2))
-    wide = zt.render(s, o, zt.Tuning(method=MEGA, node_bytes=128))
-    for other in (two, wide):
+    small = zt.render(s, o, zt.Tuning(method=MEGA, node_bytes=64))
+    for other in (two, small):
         differ = np.any(four != other, axis=2).mean()
         assert differ <= 1e-3, differ
-    assert np.array_equal(four, zt.render(s, o, zt.Tuning(method=MEGA, bvh_width=4, node_bytes=64)))
-    assert np.array_equal(wide, zt.render(s, o, zt.Tuning(method=MEGA, schedule=1, node_bytes=128)))
+    assert np.array_equal(four, zt.render(s, o, zt.Tuning(method=MEGA, bvh_width=4, node_bytes=128)))
+    assert np.array_equal(small, zt.render(s, o, zt.Tuning(method=MEGA, schedule=1, node_bytes=64)))
     assert np.array_equal(two, zt.render(s, o, zt.Tuning(method=WAVE, bvh_width=2, queue_paths=1000)))
     f64 = [zt.render(s, o, zt.Tuning(method=MEGA, precision=N.PRECISION_FP64, bvh_width=w)) for w in (2, 4)]
     assert np.array_equal(f64[0], f64[1])

@@ -257,7 +257,7 @@ void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const z
                    R* dgrad) {
   SceneView<R> view = viewOf<R>(s);
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
-  applyBvhWidth(view, tu);  // the replay must walk the tree the forward render walked
+  applyBvhWidth(view, s, tu);  // the replay must walk the tree the forward render walked
   const Params pr = makeParams(s, o);
   const uint32_t tw = 8, threads = 64;
   const uint32_t tiles_x = (pr.width + tw - 1) / tw;

@@ -38,14 +38,16 @@ zt_gpu_tuning resolveTuning(zt_ctx* ctx, const DevScene& s, const zt_render_opti
 
 // Applies zt_gpu_tuning.bvh_width to a view (0 or 4: keep the four-wide
 // nodes if the scene has them; 2: two-child traversal).
+// Views come with the four-wide tree in 128-byte nodes (nodes4q null);
+// tuning selects the two-child tree or the 64-byte nodes.
 template <typename R>
-void applyBvhWidth(ztd::SceneView<R>& v, const zt_gpu_tuning& tu) {
+void applyBvhWidth(ztd::SceneView<R>& v, const DevScene& s, const zt_gpu_tuning& tu) {
   if (tu.bvh_width != 0 && tu.bvh_width != 2 && tu.bvh_width != 4)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "bvh_width must be 0, 2 or 4");
   if (tu.node_bytes != 0 && tu.node_bytes != 64 && tu.node_bytes != 128)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "node_bytes must be 0, 64 or 128");
   if (tu.bvh_width == 2) v.nodes4 = nullptr;
-  if (tu.bvh_width == 2 || tu.node_bytes == 128) v.nodes4q = nullptr;
+  if (v.nodes4 && tu.node_bytes == 64 && s.nodes4q.p) v.nodes4q = s.nodes4q.as<ztd::DNode4Q>();
 }
 
 template <typename R>

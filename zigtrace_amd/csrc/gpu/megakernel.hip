@@ -382,8 +382,7 @@ __global__ __launch_bounds__(256) void intersectKernel(SceneView<R> sc, const zt
                                                        int mode, zt_hit* out, int* spill, int* err) {
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, kLds, spill);
-  if (mode == 2) sc.nodes4 = nullptr;   // the two-child traversal
-  if (mode == 2 || mode == 3) sc.nodes4q = nullptr;  // mode 3: four-wide 128-byte nodes
+  if (mode == 2) sc.nodes4 = nullptr;  // the two-child traversal
   int overflow = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = intersectOne<R>(sc, stk, rays[i], tmin, tmax, mode, &overflow);
@@ -503,7 +502,7 @@ template <typename R>
 void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, R* dout) {
   SceneView<R> view = viewOf<R>(s);
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);  // render.zig:157,162
-  applyBvhWidth(view, tu);
+  applyBvhWidth(view, s, tu);
   Params pr = makeParams(s, o);
   // One pixel per lane. Defaults measured on the 1080p mesh field (1.3M
   // triangles): 8x8 tiles (one wave each; many tiles per resident block
@@ -682,13 +681,15 @@ int zt_gpu_intersect(zt_ctx* ctx, uint32_t scene_id, const zt_ray* rays, uint64_
     const uint32_t threads = 256;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + threads - 1) / threads, kRayGridBlocks);
     ctx->spill.ensure(spillBytes((uint64_t)blocks * threads));
+    SceneView<float> viewQ = viewOf<float>(s);
+    if (mode == 3 && viewQ.nodes4 && s.nodes4q.p) viewQ.nodes4q = s.nodes4q.as<DNode4Q>();  // 64-byte nodes
     if (precision == ZT_PRECISION_FP64)
       hipLaunchKernelGGL(intersectKernel<double>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
                          viewOf<double>(s), drays.as<zt_ray>(), n, t_min, t_max, mode, dout.as<zt_hit>(),
                          ctx->spill.as<int>(), ctx->errflag.as<int>());
     else
       hipLaunchKernelGGL(intersectKernel<float>, dim3(blocks), dim3(threads), ldsBytes(threads), ctx->stream,
-                         viewOf<float>(s), drays.as<zt_ray>(), n, (float)t_min, (float)t_max, mode,
+                         viewQ, drays.as<zt_ray>(), n, (float)t_min, (float)t_max, mode,
                          dout.as<zt_hit>(), ctx->spill.as<int>(), ctx->errflag.as<int>());
     ZT_HIP(hipGetLastError());
     ZT_HIP(hipMemcpyAsync(out, dout.p, n * sizeof(zt_hit), hipMemcpyDeviceToHost, ctx->stream));

@@ -64,7 +64,7 @@ SceneView<R> viewOf(const DevScene& s) {
   v.nodes = s.brute_force ? nullptr : s.nodes.as<DNode>();
   v.num_nodes = s.brute_force ? 0 : s.num_nodes;
   v.nodes4 = (std::is_same<R, float>::value && !s.brute_force && s.num_nodes4) ? s.nodes4.as<DNode4>() : nullptr;
-  v.nodes4q = (v.nodes4 && s.nodes4q.p) ? s.nodes4q.as<DNode4Q>() : nullptr;
+  v.nodes4q = nullptr;  // 64-byte nodes only when a tuning asks for them
   if constexpr (std::is_same<R, float>::value) {
     v.prims = s.primsF.p;
     v.planes = s.planesF.as<Plane<float>>();

@@ -353,7 +353,7 @@ void launchWavefront(zt_ctx* ctx, const DevScene& s, const zt_render_options& o,
   ws.view = viewOf<R>(s);
   ws.pr = makeParams(s, o);
   ws.view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
-  applyBvhWidth(ws.view, tu);
+  applyBvhWidth(ws.view, s, tu);
   ws.out = dout;
   ws.first_pixel = 0;
   runWavefront<R>(ctx, std::vector<WfScene<R>>{ws}, tu);
@@ -405,7 +405,7 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
         const DevScene& s = sceneOf(ctx, scene_ids[i]);
         scenes[i].view = viewOf<R>(s);
         scenes[i].view.cam = cameraInit(s.camera, (double)opts[i].width / (double)opts[i].height);
-        applyBvhWidth(scenes[i].view, tu);
+        applyBvhWidth(scenes[i].view, s, tu);
         scenes[i].pr = makeParams(s, opts[i]);
         scenes[i].out = (R*)devOut(i);
         scenes[i].first_pixel = first;
