@@ -152,6 +152,17 @@ int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, cons
 int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, uint32_t method,
                     uint32_t precision, uint32_t probe_spp, zt_gpu_tuning* best, double* best_ms);
 
+/* Output.encode (image.zig:30-37) of `count` linear values on the device:
+ * the framebuffer of zt_gpu_render (float, or double with
+ * ZT_PRECISION_FP64) to RGB8, as zt_encode_rgb8 does on the host (f64
+ * arithmetic; the device's pow may differ from the host's in the last ulp,
+ * which moves a byte only when the value sits on a rounding boundary).
+ * `hdr_on_device` / `rgb_on_device` say where the buffers live. No reference
+ * counterpart on the device: it spares the HDR copy when only RGB8 is
+ * needed. */
+int zt_gpu_encode_rgb8(zt_ctx* ctx, const void* hdr, uint32_t precision, uint64_t count, const zt_output* out,
+                       uint8_t* rgb, int hdr_on_device, int rgb_on_device);
+
 /* traceWith (render.zig:61-102) for n caller-given rays, each with its own
  * xoshiro256++ state (4 x u64 per ray, advanced in place as the reference
  * advances `*Rng`). Radiance goes to out_rgb (n*3 doubles). Host buffers. */

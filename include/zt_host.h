@@ -76,6 +76,21 @@ int zt_bvh_build(const zt_shape* shapes, uint64_t n, zt_bvh_node* nodes_out, uin
 int zt_bvh_build_threads(const zt_shape* shapes, uint64_t n, uint32_t threads, zt_bvh_node* nodes_out,
                          uint64_t* num_nodes, uint32_t* indices_out);
 
+/* Output stage (image.zig, png.zig), on the host. -------------------------- */
+/* Output.encode for `count` linear values (image.zig:30-37): NaN -> 0,
+ * exposure, tone map, clamp to [0,1], pow(x, 1/gamma), round half away from
+ * zero to 0..255. With count = width*height*3 this is Image.toRgb8
+ * (image.zig:64-72). */
+int zt_encode_rgb8(const double* linear, uint64_t count, const zt_output* out, uint8_t* rgb);
+/* writePpm (image.zig:75-80) and writePng (png.zig:10-57: 8-bit RGB, no
+ * filtering, one IDAT of stored deflate blocks) of packed RGB8 into `dst`.
+ * The encoded length goes to *len; when dst is NULL or cap < *len nothing is
+ * written (ZT_ERR_BUFFER_TOO_SMALL unless dst is NULL). writePng rejects
+ * zero or > 2^31-1 dimensions (ZT_ERR_INVALID_DIMENSIONS) and zlib streams
+ * over 2^31-1 bytes (ZT_ERR_IMAGE_TOO_LARGE) as the reference does. */
+int zt_write_ppm(uint64_t width, uint64_t height, const uint8_t* rgb, uint8_t* dst, uint64_t cap, uint64_t* len);
+int zt_write_png(uint64_t width, uint64_t height, const uint8_t* rgb, uint8_t* dst, uint64_t cap, uint64_t* len);
+
 /* render.render (render.zig:152-180) on the GPU: validates options first
  * (ZT_ERR_INVALID_OPTIONS), uploads the scene when it changed, renders and
  * widens the fp32 framebuffer to `pixels` (width*height*3 doubles, the

@@ -26,7 +26,10 @@ enum {
   ZT_ERR_UNBOUNDED_SHAPE = -6,     /* error.UnboundedShape        bvh.zig:55      */
   ZT_ERR_TOO_MANY_PRIMITIVES = -7, /* error.TooManyPrimitives     bvh.zig:44      */
   ZT_ERR_SYNTAX = -8,              /* error.SyntaxError           scenefile.zig:31 */
-  ZT_ERR_INTERNAL = -9             /* device-side invariant violated (e.g. stack overflow) */
+  ZT_ERR_INTERNAL = -9,            /* device-side invariant violated (e.g. stack overflow) */
+  ZT_ERR_INVALID_DIMENSIONS = -10, /* error.InvalidDimensions     png.zig:12      */
+  ZT_ERR_IMAGE_TOO_LARGE = -11,    /* error.ImageTooLarge         png.zig:31      */
+  ZT_ERR_BUFFER_TOO_SMALL = -12    /* output buffer shorter than the encoding (length reported) */
 };
 
 /* material.zig:15-25 `Material` (tag + payload). */
@@ -113,6 +116,16 @@ typedef struct zt_hit {
   int32_t hit;
   int32_t prim; /* GPU results: device primitive slot hit (planes after the BVH primitives) */
 } zt_hit;
+
+/* image.zig:6-38 `Output` (and its `Tonemap` enum). Defaults: none, 2.2, 1.0
+ * (the CLI's default tonemap is aces, main.zig:47). */
+enum { ZT_TONEMAP_NONE = 0, ZT_TONEMAP_REINHARD = 1, ZT_TONEMAP_ACES = 2 };
+typedef struct zt_output {
+  uint32_t tonemap;
+  uint32_t _pad;
+  double gamma;    /* display gamma; 1 leaves values linear */
+  double exposure; /* multiplier applied before tone mapping */
+} zt_output;
 
 #ifdef __cplusplus
 }

@@ -27,10 +27,15 @@ ERR_UNBOUNDED_SHAPE = -6
 ERR_TOO_MANY_PRIMITIVES = -7
 ERR_SYNTAX = -8
 ERR_INTERNAL = -9
+ERR_INVALID_DIMENSIONS = -10
+ERR_IMAGE_TOO_LARGE = -11
+ERR_BUFFER_TOO_SMALL = -12
 ERROR_NAMES = {
     ERR_INVALID_OPTIONS: "InvalidOptions", ERR_OUT_OF_MEMORY: "OutOfMemory", ERR_HIP: "HipError",
     ERR_BAD_SCENE: "BadScene", ERR_INVALID_ARGUMENT: "InvalidArgument", ERR_UNBOUNDED_SHAPE: "UnboundedShape",
     ERR_TOO_MANY_PRIMITIVES: "TooManyPrimitives", ERR_SYNTAX: "SyntaxError", ERR_INTERNAL: "Internal",
+    ERR_INVALID_DIMENSIONS: "InvalidDimensions", ERR_IMAGE_TOO_LARGE: "ImageTooLarge",
+    ERR_BUFFER_TOO_SMALL: "BufferTooSmall",
 }
 
 LAMBERTIAN, METAL, DIELECTRIC, EMISSIVE = 0, 1, 2, 3
@@ -55,6 +60,8 @@ HIT = np.dtype([("t", "<f8"), ("point", "<f8", (3,)), ("normal", "<f8", (3,)), (
 TUNING = np.dtype([("method", "<u4"), ("precision", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"),
                    ("block_size", "<u4"), ("queue_paths", "<u4"), ("variant", "<u4"), ("stack_entries", "<u4"),
                    ("bvh_width", "<u4"), ("schedule", "<u4"), ("refill", "<u4"), ("node_bytes", "<u4")])
+OUTPUT = np.dtype([("tonemap", "<u4"), ("_pad", "<u4"), ("gamma", "<f8"), ("exposure", "<f8")])
+TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 STATS = np.dtype([("kernel_ms", "<f8"), ("upload_ms", "<f8"), ("launches", "<u8"), ("paths", "<u8"),
                   ("extend_rays", "<u8"), ("shadow_rays", "<u8")])
 SCENE_INFO = np.dtype([("num_materials", "<u8"), ("num_shapes", "<u8"), ("num_planes", "<u8"), ("num_nodes", "<u8"),
@@ -62,7 +69,7 @@ SCENE_INFO = np.dtype([("num_materials", "<u8"), ("num_shapes", "<u8"), ("num_pl
                        ("spp", "<u8"), ("max_depth", "<u8"), ("built", "<u4"), ("_pad", "<u4")])
 
 for _dt, _size in ((MATERIAL, 40), (SHAPE, 80), (CAMERA, 104), (BACKGROUND, 56), (NODE, 56), (LIGHT, 40),
-                   (OPTIONS, 64), (RAY, 48), (HIT, 72), (TUNING, 48), (STATS, 48), (SCENE_INFO, 88)):
+                   (OPTIONS, 64), (RAY, 48), (HIT, 72), (TUNING, 48), (STATS, 48), (OUTPUT, 24), (SCENE_INFO, 88)):
     assert _dt.itemsize == _size, (_dt, _size)
 
 
@@ -109,6 +116,8 @@ _SIGS = {
     "zt_gpu_render": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "zt_gpu_render_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int]),
+    "zt_gpu_encode_rgb8": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_int]),
     "zt_gpu_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                   C.c_void_p]),
     "zt_gpu_trace_rays": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
@@ -138,6 +147,9 @@ _SIGS = {
     "zt_scene_get_view": (C.c_int, [C.c_void_p, C.c_void_p]),
     "zt_scene_parse": (C.c_int, [C.c_char_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint64]),
     "zt_bvh_build": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "zt_encode_rgb8": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "zt_write_ppm": (C.c_int, [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "zt_write_png": (C.c_int, [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "zt_bvh_build_threads": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_default_context": (C.c_int, [C.c_void_p]),

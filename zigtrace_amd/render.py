@@ -150,6 +150,28 @@ class Context:
                                             N.ptr(rec), C.byref(ms)))
         return Tuning(**{k: int(rec[k][0]) for k in N.TUNING.names}), ms.value
 
+    def encode_rgb8(self, hdr, out=None, count: int | None = None, precision: int = N.PRECISION_FP32,
+                    rgb_device_ptr: int | None = None) -> np.ndarray | None:
+        """Output.encode on the device. `hdr` is a host array or a device address (then give
+        `count` and `precision`); returns host RGB8 unless `rgb_device_ptr` is given."""
+        from .image import Output
+        rec = (out or Output()).record()
+        on_dev = not isinstance(hdr, np.ndarray)
+        if on_dev:
+            src, n = C.c_void_p(hdr), int(count)
+        else:
+            arr = np.ascontiguousarray(hdr)
+            precision = N.PRECISION_FP64 if arr.dtype == np.float64 else N.PRECISION_FP32
+            arr = arr.astype(np.float64 if precision == N.PRECISION_FP64 else np.float32, copy=False)
+            src, n = N.ptr(arr), arr.size
+        if rgb_device_ptr is not None:
+            self._check(N.lib().zt_gpu_encode_rgb8(self._h, src, precision, n, N.ptr(rec), C.c_void_p(rgb_device_ptr),
+                                                   int(on_dev), 1))
+            return None
+        rgb = np.zeros(n, dtype=np.uint8)
+        self._check(N.lib().zt_gpu_encode_rgb8(self._h, src, precision, n, N.ptr(rec), N.ptr(rgb), int(on_dev), 0))
+        return rgb.reshape(hdr.shape) if not on_dev else rgb
+
     def render_batch(self, scene_ids, options, tuning: Tuning | None = None, outs=None):
         """Renders several scenes in one wavefront pass; returns host images."""
         t = tuning or Tuning(method=N.METHOD_WAVEFRONT)
