@@ -23,8 +23,7 @@ template <typename R>
 struct Path {
   V3<R> ro, rd;  // extension ray; ro is also the shadow-ray origin
   V3<R> radiance, throughput;
-  V3<R> prev_origin;
-  R prev_pdf;
+  R prev_pdf;  // BSDF pdf of the bounce that cast rd from ro (0: no MIS weight)
   V3<R> sh_d;        // pending shadow ray direction
   R sh_tmax;         // distance * (1 - 1e-6)
   V3<R> sh_contrib;  // throughput * directLight, added if unoccluded
@@ -279,7 +278,6 @@ ZT_D int beginPath(const SceneView<R>& sc, const Params& pr, uint32_t x, uint32_
   cameraRay<R>(sc.cam, x, y, (double)pr.width, (double)pr.height, rng, &st.ro, &st.rd);
   st.radiance = splat(R(0));
   st.throughput = splat(R(1));
-  st.prev_origin = splat(R(0));
   st.prev_pdf = R(0);
   st.self = -1;
   st.sh_excl = -1;
@@ -303,7 +301,8 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
   const Mat<R> m = sc.mats[s.material];
   const bool emissive_front = m.kind == ZT_EMISSIVE && s.front;
   if (emissive_front && maxComponent(m.color) > 0) {
-    const R w = st.prev_pdf > 0 ? powerWeight(st.prev_pdf, pdfHit(sc, st.prev_origin, s.p, h.prim)) : R(1);
+    // The previous vertex is this ray's origin (shade sets ro to the hit point).
+    const R w = st.prev_pdf > 0 ? powerWeight(st.prev_pdf, pdfHit(sc, st.ro, s.p, h.prim)) : R(1);
     const V3<R> x = scale(mul(st.throughput, m.color), w);
     st.radiance = add(st.radiance, x);
     if (obs) obs->emission(x, scale(st.throughput, w), s.material);
@@ -335,7 +334,6 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
     if (obs && (m.kind == ZT_LAMBERTIAN || m.kind == ZT_METAL)) obs->scattered(s.material);
     st.throughput = mul(st.throughput, att);
     st.prev_pdf = (pr.use_nee && lambertian) ? fmax(dot(dir, s.n), R(0)) / K<R>::pi : R(0);
-    st.prev_origin = s.p;
     st.rd = dir;
     if ((uint32_t)st.depth + 1 >= 4) {  // Russian roulette, render.zig:94-99
       const R p = fmin(maxComponent(st.throughput), R(0.95));

@@ -424,6 +424,8 @@ struct Trav4 {
     sp += m;
   }
   // One pop or one interior node. Returns true when the BVH is exhausted.
+  // Q: the nodes are the 64-byte DNode4Q (else the 128-byte DNode4).
+  template <bool Q>
   ZT_D bool inner(const SceneView<float>& sc, const LaneStack& stk, const Query4& q, int* overflow) {
     if (cur == kNoChild) {
       if (sp == 0) {
@@ -442,7 +444,7 @@ struct Trav4 {
     }
     float4 lx, ly, lz, hx, hy, hz;
     int c0, c1, c2, c3;
-    if (sc.nodes4q) {
+    if constexpr (Q) {
       const uint4* p = (const uint4*)(sc.nodes4q + cur);
       const uint4 a = p[0], b = p[1], c = p[2];
       const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
@@ -515,15 +517,15 @@ struct Trav4 {
 // One four-wide query to the end, while-while: a lane pops and visits
 // interior nodes until it holds a leaf (or has nothing left), then tests that
 // leaf, so a wave runs node work and leaf work in separate passes.
-ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
-                             float tmax, int self, int excl, bool anyhit, int* overflow) {
-  const Query4 q{o, d, tmin, self, excl, anyhit};
+template <bool Q>
+ZT_D HitRec<float> traverse4Impl(const SceneView<float>& sc, const LaneStack& stk, const Query4& q, float tmax,
+                                 int* overflow) {
   Trav4 t;
   t.start(q, tmax);
   for (;;) {
     bool empty = false;
     while (!t.atLeaf()) {
-      if (t.inner(sc, stk, q, overflow)) {
+      if (t.inner<Q>(sc, stk, q, overflow)) {
         empty = true;
         break;
       }
@@ -531,6 +533,11 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
     if (empty || t.leaf(sc, q)) break;
   }
   return t.finish(sc, q);
+}
+ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V3<float> o, V3<float> d, float tmin,
+                             float tmax, int self, int excl, bool anyhit, int* overflow) {
+  const Query4 q{o, d, tmin, self, excl, anyhit};
+  return sc.nodes4q ? traverse4Impl<true>(sc, stk, q, tmax, overflow) : traverse4Impl<false>(sc, stk, q, tmax, overflow);
 }
 
 // One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force

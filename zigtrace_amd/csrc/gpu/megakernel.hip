@@ -154,7 +154,7 @@ struct ManyScenes {
   ZT_D float* output(uint32_t si) const { return jobs[si].out; }
 };
 
-template <typename Src>
+template <bool Q, typename Src>
 ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap,
                              uint32_t refill, int* spill, uint32_t* band_next, int* err) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
@@ -174,7 +174,6 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   V3<float> sum = splat(0.0f);
   Path<float> st;
   Trav4 tr;
-  HitRec<float> hit{0.0f, -1};
   int overflow = 0;
   for (;;) {
     // Pixels for lanes without one, from the wave's tile (then the next tile).
@@ -225,7 +224,10 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       const SceneView<float>& sc = src.view(si);
       const Params& pr = src.params(si);
       for (;;) {
-        if (phase == kResolve) phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc, pr, hit, rng, st);
+        if (phase == kResolve) {
+          const HitRec<float> hit{tr.closest, tr.best};  // what tr.finish left
+          phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc, pr, hit, rng, st);
+        }
         if (phase == kFinish) {
           if (isFinite(st.radiance)) sum = add(sum, st.radiance);  // render.zig:141-142
           ++sample;
@@ -273,10 +275,10 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       if (n_leaf >= n_node) {
         if (at_leaf) ended = tr.leaf(src.view(si), q);
       } else if (tracing && !at_leaf) {
-        ended = tr.inner(src.view(si), stk, q, &overflow);
+        ended = tr.template inner<Q>(src.view(si), stk, q, &overflow);
       }
       if (ended) {
-        hit = tr.finish(src.view(si), q);
+        (void)tr.finish(src.view(si), q);
         tracing = false;
         phase = kResolve;
       }
@@ -285,20 +287,20 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   if (overflow) atomicOr(err, 1);
 }
 
-template <int W>
+template <int W, bool Q>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
     OneScene src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
     uint32_t* band_next, int* err) {
-  megakernelWaveBody(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
+  megakernelWaveBody<Q>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
 }
 
 // Many scenes in one persistent launch (zt_gpu_render_batch): the same
 // kernel over the concatenated tiles of every scene.
-template <int W>
+template <int W, bool Q>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWaveBatch(
     ManyScenes src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
     uint32_t* band_next, int* err) {
-  megakernelWaveBody(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
+  megakernelWaveBody<Q>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
 }
 
 __global__ void clearBands(uint32_t* band_next) {
@@ -315,7 +317,6 @@ ZT_D void traceOne(const SceneView<R>& sc, const Params& pr, const LaneStack& st
   st.rd = mk((R)rays[i].dir[0], (R)rays[i].dir[1], (R)rays[i].dir[2]);
   st.radiance = splat(R(0));
   st.throughput = splat(R(1));
-  st.prev_origin = splat(R(0));
   st.prev_pdf = R(0);
   st.self = -1;
   st.sh_excl = -1;
@@ -451,13 +452,23 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   if (ntiles == 0) return;
   if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
   const bool many = jobs.size() > 1;
+  // 64-byte nodes when the caller asked for them (then every scene has them).
+  const bool quant = jobs[0].view.nodes4q != nullptr;
+  for (const WaveJob& j : jobs)
+    if ((j.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
   const uint32_t threads = 64;
   const size_t lds = stackBytes(cap, (int)threads);
-  static const void* const kOne[4] = {(const void*)megakernelWave<1>, (const void*)megakernelWave<4>,
-                                      (const void*)megakernelWave<5>, (const void*)megakernelWave<6>};
-  static const void* const kMany[4] = {(const void*)megakernelWaveBatch<1>, (const void*)megakernelWaveBatch<4>,
-                                       (const void*)megakernelWaveBatch<5>, (const void*)megakernelWaveBatch<6>};
-  const void* fn = (many ? kMany : kOne)[variant - 1];
+  static const void* const kOne[2][4] = {
+      {(const void*)megakernelWave<1, false>, (const void*)megakernelWave<4, false>,
+       (const void*)megakernelWave<5, false>, (const void*)megakernelWave<6, false>},
+      {(const void*)megakernelWave<1, true>, (const void*)megakernelWave<4, true>,
+       (const void*)megakernelWave<5, true>, (const void*)megakernelWave<6, true>}};
+  static const void* const kMany[2][4] = {
+      {(const void*)megakernelWaveBatch<1, false>, (const void*)megakernelWaveBatch<4, false>,
+       (const void*)megakernelWaveBatch<5, false>, (const void*)megakernelWaveBatch<6, false>},
+      {(const void*)megakernelWaveBatch<1, true>, (const void*)megakernelWaveBatch<4, true>,
+       (const void*)megakernelWaveBatch<5, true>, (const void*)megakernelWaveBatch<6, true>}};
+  const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
   ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
   ctx->bands.ensure(8 * sizeof(uint32_t));
@@ -466,25 +477,18 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   int* err = ctx->errflag.as<int>();
   const uint32_t nt = (uint32_t)ntiles;
   hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
+  // All instantiations take the same arguments; launch the one selected above.
   if (!many) {
-    const OneScene src{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
-    switch (variant) {
-      case 1: hipLaunchKernelGGL(megakernelWave<1>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      case 2: hipLaunchKernelGGL(megakernelWave<4>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      case 3: hipLaunchKernelGGL(megakernelWave<5>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      default: hipLaunchKernelGGL(megakernelWave<6>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-    }
+    OneScene src{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
+    void* args[] = {&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands, &err};
+    ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
   } else {
     ctx->jobs.ensure(jobs.size() * sizeof(WaveJob));
     ZT_HIP(hipMemcpyAsync(ctx->jobs.p, jobs.data(), jobs.size() * sizeof(WaveJob), hipMemcpyHostToDevice,
                           ctx->stream));
-    const ManyScenes src{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
-    switch (variant) {
-      case 1: hipLaunchKernelGGL(megakernelWaveBatch<1>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      case 2: hipLaunchKernelGGL(megakernelWaveBatch<4>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      case 3: hipLaunchKernelGGL(megakernelWaveBatch<5>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-      default: hipLaunchKernelGGL(megakernelWaveBatch<6>, dim3(grid), dim3(threads), lds, ctx->stream, src, tw, th, nt, cap, refill, spill, bands, err); break;
-    }
+    ManyScenes src{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
+    void* args[] = {&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands, &err};
+    ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
   }
   ZT_HIP(hipGetLastError());
   ctx->stats.launches += 2;
