@@ -172,24 +172,26 @@ ZT_D float entryT(R t) {
 // Tests the primitives of leaf `code` in slot order (scene.zig:100-104 via
 // bvh.zig:108-117), shrinking *closest. Returns true when an any-hit query
 // found its hit.
+// Primitive k against a ray whose interval is (tmin, tmax); the fp32 path
+// skips the shadow ray's target and treats the ray's own primitive by
+// identity (a sphere then yields only the far end of its chord).
+template <typename R>
+ZT_D bool primHit(const SceneView<R>& sc, int k, V3<R> o, V3<R> d, R tmin, R tmax, int self, int excl, R* t) {
+  constexpr bool kIdentity = !K<R>::exact;
+  if (kIdentity && k == excl) return false;
+  const Prim<R> p = loadPrim<R>(sc, k);
+  if (metaKind(p.meta) == kKindTriangle) return (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, tmax, t);
+  return (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, tmax, t) : sphereHit<R>(p, o, d, tmin, tmax, t);
+}
+
 template <typename R>
 ZT_D bool leafHits(const SceneView<R>& sc, int code, V3<R> o, V3<R> d, R tmin, R* closest, int* best, int self,
                    int excl, bool anyhit) {
-  constexpr bool kIdentity = !K<R>::exact;
   const int v = -1 - code;
   const int first = v >> 4, count = v & 15;
   for (int k = first; k < first + count; ++k) {
-    if (kIdentity && k == excl) continue;
-    const Prim<R> p = loadPrim<R>(sc, k);
     R t;
-    bool h;
-    if (metaKind(p.meta) == kKindTriangle) {
-      h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, *closest, &t);
-    } else {
-      h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, *closest, &t)
-                                   : sphereHit<R>(p, o, d, tmin, *closest, &t);
-    }
-    if (h) {
+    if (primHit<R>(sc, k, o, d, tmin, *closest, self, excl, &t)) {
       *closest = t;
       *best = k;
       if (anyhit) return true;
@@ -324,17 +326,8 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
   R closest = tmax;
   int best = -1;
   for (int k = 0; k < sc.num_prims; ++k) {
-    if (kIdentity && k == excl) continue;
-    const Prim<R> p = loadPrim<R>(sc, k);
     R t;
-    bool h;
-    if (metaKind(p.meta) == kKindTriangle) {
-      h = (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, closest, &t);
-    } else {
-      h = (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, closest, &t)
-                                   : sphereHit<R>(p, o, d, tmin, closest, &t);
-    }
-    if (h) {
+    if (primHit<R>(sc, k, o, d, tmin, closest, self, excl, &t)) {
       closest = t;
       best = k;
       if (anyhit) return HitRec<R>{closest, best};
