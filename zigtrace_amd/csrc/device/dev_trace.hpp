@@ -416,11 +416,13 @@ struct Trav4 {
     }
     sp += m;
   }
-  // One pop or one interior node. Returns true when the BVH is exhausted.
+  // One interior node, after popping until an entry survives pruning when
+  // none is held (so a popped node costs no separate step). Returns true
+  // when the BVH is exhausted; a popped leaf is held for the leaf step.
   // Q: the nodes are the 64-byte DNode4Q (else the 128-byte DNode4).
   template <bool Q>
   ZT_D bool inner(const SceneView<float>& sc, const LaneStack& stk, const Query4& q, int* overflow) {
-    if (cur == kNoChild) {
+    while (cur == kNoChild) {
       if (sp == 0) {
         if (nsp == 0) return true;
         const int half = stk.cap >> 1;
@@ -433,8 +435,8 @@ struct Trav4 {
       }
       --sp;
       if (stk.tval[sp * stk.stride] <= closest) cur = stk.code[sp * stk.stride];
-      return false;
     }
+    if (cur < 0) return false;
     float4 lx, ly, lz, hx, hy, hz;
     int c0, c1, c2, c3;
     if constexpr (Q) {

@@ -331,9 +331,11 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       continue;
     }
     // Traverse in wave-uniform passes until enough lanes wait for work: a
-    // leaf pass once at least as many lanes hold an untested leaf as still
-    // walk interior nodes, otherwise a node pass (one pop or node each), so
-    // no pass pays for the other's code.
+    // leaf pass once the lanes holding an untested leaf are at least half as
+    // many as those still walking interior nodes (the leaf pass deals the
+    // primitives out to every lane, so it pays to run it early), otherwise a
+    // node pass (pops, then one node each), so no pass pays for the other's
+    // code.
     for (;;) {
       const Query4 q{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh};
       const bool at_leaf = tracing && tr.atLeaf();
@@ -343,7 +345,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       const uint32_t waiting = (uint32_t)__popcll(__ballot(!tracing && (has || !tiles_done)));
       if (waiting >= refill) break;
       bool ended = false;
-      if (n_leaf >= n_node) {
+      if (2 * n_leaf >= n_node) {
         ended = leafPassWave(src, si, tr, q, at_leaf, leaf_keys, leaf_owner);
       } else if (tracing && !at_leaf) {
         ended = tr.template inner<Q>(src.view(si), stk, q, &overflow);
