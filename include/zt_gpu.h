@@ -141,7 +141,7 @@ int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scene_ids, cons
                         const zt_gpu_tuning* tuning, void* const* outs, int out_on_device);
 
 /* Autotuner (no reference counterpart). Renders the scene at the options'
- * resolution with min(spp, probe_spp) samples (probe_spp 0 = 4) under each
+ * resolution with min(spp, probe_spp) samples (probe_spp 0 = 16) under each
  * candidate launch configuration of `method` (AUTO: both methods) --
  * megakernel: build variant x LDS stack entries x tile shape; wavefront:
  * block size x path-slot pool x LDS stack entries -- times them with HIP

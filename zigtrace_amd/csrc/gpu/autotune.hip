@@ -119,7 +119,7 @@ extern "C" int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_o
     DevScene& s = sceneOf(ctx, scene_id);
     setDevice(ctx);
     zt_render_options probe = *opts;
-    probe.spp = std::min<uint64_t>(opts->spp, probe_spp ? probe_spp : 4);
+    probe.spp = std::min<uint64_t>(opts->spp, probe_spp ? probe_spp : 16);
     const uint64_t pixels = (uint64_t)probe.width * probe.height;
     const size_t elem = precision == ZT_PRECISION_FP64 ? sizeof(double) : sizeof(float);
     ctx->scratch.ensure(pixels * 3 * elem);
