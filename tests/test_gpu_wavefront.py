@@ -1,5 +1,7 @@
 """The wavefront renderer produces the megakernel's image bit for bit, for
 any path-pool size (slot reuse across pixels), block size and precision."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -142,3 +144,55 @@ def test_autotune_records_a_configuration_with_identical_output():
         assert np.array_equal(ctx.render(sid, o, None), default)
     finally:
         ctx.close()
+
+
+def _with_large_leaves(nodes: np.ndarray, cap: int) -> np.ndarray:
+    """The host BVH with every subtree of at most `cap` primitives turned into
+    one leaf over its (contiguous) primitive range -- the shape of the leaves
+    the reference's build emits at its depth cap (bvh.zig:150-152)."""
+    out = nodes.copy()
+    first = out["first"].astype(np.int64).copy()
+    count = out["count"].astype(np.int64).copy()
+    for i in range(len(out) - 1, -1, -1):  # children follow their parent
+        if out["count"][i] == 0:
+            left = int(out["first"][i])
+            first[i] = min(first[left], first[left + 1])
+            count[i] = count[left] + count[left + 1]
+    for i in range(len(out)):
+        if out["count"][i] == 0 and count[i] <= cap:
+            out["first"][i] = first[i]
+            out["count"][i] = count[i]
+    return out
+
+
+@pytest.mark.parametrize("cap", [9, 15, 40])
+def test_large_leaves_trace_like_brute_force_and_render_identically(gpu_ctx, cap):
+    """Leaves of up to 15 primitives (longer ones are split into synthetic
+    nodes at upload) go through the wave kernel's parallel leaf pass: closest
+    hits equal brute force and every schedule renders the same bits."""
+    from zigtrace_amd import workloads
+    from .helpers import aimed_rays
+    wl = workloads.mesh_field(subdivisions=2, meshes=6, spp=2, width=40, height=24)
+    arrays = wl.scene.view_arrays()
+    nodes = _with_large_leaves(arrays["nodes"], cap)
+    assert nodes["count"].max() > 4
+    view = wl.scene.view()
+    view.nodes = nodes.ctypes.data
+    sid_c = C.c_uint32()
+    N.check(N.lib().zt_gpu_scene_upload(gpu_ctx.handle, C.byref(view), C.byref(sid_c)), gpu_ctx.handle)
+    sid = sid_c.value
+    rng = np.random.default_rng(cap)
+    rays = aimed_rays(rng, arrays["shapes"], 2000)
+    brute = gpu_ctx.intersect(sid, rays, 1e-4, 1e9, mode=1)
+    for mode in (0, 2):
+        got = gpu_ctx.intersect(sid, rays, 1e-4, 1e9, mode=mode)
+        assert np.array_equal(got["hit"], brute["hit"])
+        h = got["hit"] == 1
+        assert h.sum() > 100
+        assert np.array_equal(got["t"][h], brute["t"][h]) and np.array_equal(got["material"][h], brute["material"][h])
+    o = wl.options
+    base = gpu_ctx.render(sid, o, zt.Tuning(method=MEGA, schedule=1))
+    for refill in (1, 32):
+        img = gpu_ctx.render(sid, o, zt.Tuning(method=MEGA, schedule=2, refill=refill))
+        assert np.array_equal(base, img), refill
+    gpu_ctx.release(sid)
