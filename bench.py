@@ -26,6 +26,7 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method auto|megakernel|wavefront]
                        [--no-cpu] [--no-extra] [--no-tune] [--spp S]
+       python bench.py --only batch|adjoint [--adjoint-steps S]   (one side leg alone, for profiling)
        python bench.py --copy-probe BYTES   (one device-to-device copy of BYTES, for
                                              calibrating HBM traffic counters)
 """
@@ -187,6 +188,9 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner")
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel (64 is the benchmark; less for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--only", choices=["batch", "adjoint"], default=None,
+                    help="run one side leg alone and print its record (for profiling its kernels)")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="Adam steps of the adjoint leg")
     ap.add_argument("--schedule", type=int, default=0, help="megakernel schedule with --no-tune (0/1 pixel, 2 wave)")
     ap.add_argument("--refill", type=int, default=0, help="wave-scheduled refill threshold with --no-tune")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries with --no-tune")
@@ -203,6 +207,11 @@ def main() -> int:
     import zigtrace_amd as zt
     from zigtrace_amd import _native as N
     from zigtrace_amd import workloads as W
+
+    if args.only:
+        leg = bench_batch(zt, N, W, steps=1) if args.only == "batch" else bench_adjoint(zt, N, W, args.adjoint_steps)
+        print(json.dumps({args.only: leg}), flush=True)
+        return 0
 
     t0 = time.time()
     w = W.mesh_field(spp=args.spp)
@@ -276,7 +285,7 @@ def main() -> int:
         extra["batch"] = bench_batch(zt, N, W, steps=1)
         log(f"[bench] batch leg {time.time() - t:.1f}s: {extra['batch']}")
         t = time.time()
-        extra["adjoint"] = bench_adjoint(zt, N, W, steps=200)
+        extra["adjoint"] = bench_adjoint(zt, N, W, steps=args.adjoint_steps)
         log(f"[bench] adjoint leg {time.time() - t:.1f}s: {extra['adjoint']}")
     line = {
         "metric": "Mpaths/s @1080p 64spp (plain vs wavefront-batched); fwd+adjoint iters/s",
