@@ -219,7 +219,8 @@ def main() -> int:
     log(f"[bench] workload {w.description}: built in {time.time() - t0:.1f}s")
     ctx = zt.Context(0)
     sid = ctx.upload(w.scene)
-    log(f"[bench] uploaded in {ctx.stats()['upload_ms']:.0f} ms")
+    upload_ms = ctx.stats()["upload_ms"]
+    log(f"[bench] uploaded in {upload_ms:.0f} ms")
     frame_bytes = o.width * o.height * 3 * 4
     dout = ctx.alloc(frame_bytes)
     paths = o.width * o.height * o.spp
@@ -299,6 +300,7 @@ def main() -> int:
         "methods": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
         "mrays_s": round(r["mpaths_s"] * (rays_per_path or fixture_bytes_per_path()[1]), 2),
+        "scene_upload_ms": round(upload_ms, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,
