@@ -1,8 +1,9 @@
 // Wave-level pieces of the kernels that walk the four-wide BVH with one ray
 // per lane under wave-uniform passes (the wave-scheduled megakernel). `Src`
-// gives the scene view of a lane's scene index: `src.view(si)`. The workgroup must be exactly one wave: the
-// leaf pass orders its LDS traffic with __syncthreads, which is then a
-// wave-local barrier that every pass reaches together.
+// gives the scene view of a lane's scene index: `src.view(si)`. The
+// workgroup must be exactly one wave: the leaf pass orders its LDS traffic
+// with __syncthreads, which is then a wave-local barrier that every pass
+// reaches together.
 #pragma once
 
 #include "dev_trace.hpp"
