@@ -26,7 +26,8 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method auto|megakernel|wavefront]
                        [--no-cpu] [--no-extra] [--no-tune] [--spp S]
-       python bench.py --only batch|adjoint [--adjoint-steps S]   (one side leg alone, for profiling)
+       python bench.py --only batch|adjoint|configs [--adjoint-steps S]   (one side leg alone: the
+                                             batch, the adjoint, or the two smaller configurations)
        python bench.py --copy-probe BYTES   (one device-to-device copy of BYTES, for
                                              calibrating HBM traffic counters)
 """
@@ -153,6 +154,35 @@ def bench_batch(zt, N, W, steps: int):
             "prims_total": int(sum(w.scene.info()["num_shapes"] for w in wls))}
 
 
+def bench_configs(zt, N, W, steps: int = 5):
+    """The smaller configurations (the shipped-style Cornell box at 400x400, 64 spp; the
+    sphere field at 600x340, 16 spp): megakernel Mpaths/s (autotuned, median of `steps`) and
+    the oracle's Mpaths/s on the host's cores beside it."""
+    import numpy as np
+    out = {}
+    for name, wl in (("cornell_400x400_64spp", W.cornell()), ("sphere_field_600x340_16spp", W.sphere_field())):
+        o = wl.options
+        ctx = zt.Context(0)
+        sid = ctx.upload(wl.scene)
+        d = ctx.alloc(o.width * o.height * 12)
+        tu, _ = ctx.autotune(sid, o, method=N.METHOD_MEGAKERNEL)
+        ctx.render_device(sid, o, tu, d)
+        kms = []
+        for _ in range(steps):
+            ctx.render_device(sid, o, tu, d)
+            kms.append(ctx.stats()["kernel_ms"])
+        ctx.free(d)
+        ctx.close()
+        paths = o.width * o.height * o.spp
+        cpu, _, rays = cpu_baseline(wl, target_seconds=6.0)
+        gpu = paths / float(np.median(kms)) / 1e3
+        out[name] = {"paths": paths, "gpu_mpaths_s": round(gpu, 1), "gpu_kernel_ms_median": round(float(np.median(kms)), 3),
+                     "gpu_mrays_s": round(gpu * rays, 1), "cpu": cpu, "gpu_over_cpu": round(gpu / cpu["value"], 1),
+                     "tuning": {k: v for k, v in tu.__dict__.items() if v}}
+        log(f"[bench] {name}: {out[name]}")
+    return out
+
+
 def bench_adjoint(zt, N, W, steps: int):
     """Adam inverse rendering on the differentiable box: forward + backward per iteration."""
     import numpy as np
@@ -188,7 +218,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner")
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel (64 is the benchmark; less for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--only", choices=["batch", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["batch", "adjoint", "configs"], default=None,
                     help="run one side leg alone and print its record (for profiling its kernels)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="Adam steps of the adjoint leg")
     ap.add_argument("--schedule", type=int, default=0, help="megakernel schedule with --no-tune (0/1 pixel, 2 wave)")
@@ -209,7 +239,12 @@ def main() -> int:
     from zigtrace_amd import workloads as W
 
     if args.only:
-        leg = bench_batch(zt, N, W, steps=1) if args.only == "batch" else bench_adjoint(zt, N, W, args.adjoint_steps)
+        if args.only == "batch":
+            leg = bench_batch(zt, N, W, steps=1)
+        elif args.only == "adjoint":
+            leg = bench_adjoint(zt, N, W, args.adjoint_steps)
+        else:
+            leg = bench_configs(zt, N, W)
         print(json.dumps({args.only: leg}), flush=True)
         return 0
 
