@@ -84,6 +84,10 @@ typedef struct zt_gpu_tuning {
                              1..64 (0 = 32) */
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
+  uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,
+                             2 = most expensive first (a one-sample probe render counts the rays of every tile,
+                             then the tiles of each band are sorted by that count), 0 = 2 when the frame (or
+                             batch) has at least 16 samples per pixel on average, else 1 */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

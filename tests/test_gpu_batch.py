@@ -45,6 +45,7 @@ def test_batch_fp64_and_repeated_scene(gpu_ctx):
 
 @pytest.mark.parametrize("tuning", [
     dict(), dict(refill=8, tile_w=16, tile_h=8), dict(variant=1, stack_entries=8), dict(method=N.METHOD_MEGAKERNEL),
+    dict(tile_order=2), dict(tile_order=2, tile_w=16, tile_h=16, refill=40),
 ])
 def test_wave_megakernel_batch_equals_individual_renders(gpu_ctx, tuning):
     ws = W.batch_scenes(count=12, seed=9, min_prims=50, max_prims=20000, resolutions=(8, 16, 40, 64),

@@ -146,6 +146,25 @@ def test_autotune_records_a_configuration_with_identical_output():
         ctx.close()
 
 
+def test_longest_first_tile_order_is_bit_identical():
+    """Tiles taken most expensive first (after the one-sample cost probe) or in scan
+    order give the same image as the one-pixel-per-lane kernel: only the assignment of
+    tiles to waves changes. At 20 spp the default orders by cost."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=20, width=70, height=37)
+    s, o = wl.scene, wl.options
+    base = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1))
+    for order in (0, 1, 2):
+        for tw, th in ((8, 8), (13, 5), (64, 64)):
+            img = zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_order=order, tile_w=tw, tile_h=th))
+            assert np.array_equal(base, img), (order, tw, th)
+    small = zt.Options(9, 7, 3, 6, seed=2)
+    a = zt.render(s, small, zt.Tuning(method=MEGA, schedule=1))
+    assert np.array_equal(a, zt.render(s, small, zt.Tuning(method=MEGA, schedule=2, tile_order=2)))
+    with pytest.raises(zt.ZTError):
+        zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_order=3))
+
+
 def _with_large_leaves(nodes: np.ndarray, cap: int) -> np.ndarray:
     """The host BVH with every subtree of at most `cap` primitives turned into
     one leaf over its (contiguous) primitive range -- the shape of the leaves

@@ -59,7 +59,8 @@ HIT = np.dtype([("t", "<f8"), ("point", "<f8", (3,)), ("normal", "<f8", (3,)), (
                 ("material", "<u4"), ("hit", "<i4"), ("prim", "<i4")])
 TUNING = np.dtype([("method", "<u4"), ("precision", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"),
                    ("block_size", "<u4"), ("queue_paths", "<u4"), ("variant", "<u4"), ("stack_entries", "<u4"),
-                   ("bvh_width", "<u4"), ("schedule", "<u4"), ("refill", "<u4"), ("node_bytes", "<u4")])
+                   ("bvh_width", "<u4"), ("schedule", "<u4"), ("refill", "<u4"), ("node_bytes", "<u4"),
+                   ("tile_order", "<u4")])
 OUTPUT = np.dtype([("tonemap", "<u4"), ("_pad", "<u4"), ("gamma", "<f8"), ("exposure", "<f8")])
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 STATS = np.dtype([("kernel_ms", "<f8"), ("upload_ms", "<f8"), ("launches", "<u8"), ("paths", "<u8"),
@@ -69,7 +70,7 @@ SCENE_INFO = np.dtype([("num_materials", "<u8"), ("num_shapes", "<u8"), ("num_pl
                        ("spp", "<u8"), ("max_depth", "<u8"), ("built", "<u4"), ("_pad", "<u4")])
 
 for _dt, _size in ((MATERIAL, 40), (SHAPE, 80), (CAMERA, 104), (BACKGROUND, 56), (NODE, 56), (LIGHT, 40),
-                   (OPTIONS, 64), (RAY, 48), (HIT, 72), (TUNING, 48), (STATS, 48), (OUTPUT, 24), (SCENE_INFO, 88)):
+                   (OPTIONS, 64), (RAY, 48), (HIT, 72), (TUNING, 52), (STATS, 48), (OUTPUT, 24), (SCENE_INFO, 88)):
     assert _dt.itemsize == _size, (_dt, _size)
 
 

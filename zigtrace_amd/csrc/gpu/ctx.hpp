@@ -15,7 +15,7 @@
 #include "zt_gpu.h"
 
 // The ABI records keep the sizes the bindings are written against.
-static_assert(sizeof(zt_gpu_tuning) == 48, "zt_gpu_tuning layout");
+static_assert(sizeof(zt_gpu_tuning) == 52, "zt_gpu_tuning layout");
 static_assert(sizeof(zt_gpu_stats) == 48, "zt_gpu_stats layout");
 static_assert(sizeof(zt_render_options) == 64, "zt_render_options layout");
 static_assert(sizeof(zt_ray) == 48 && sizeof(zt_hit) == 72, "zt_ray / zt_hit layout");
@@ -132,6 +132,7 @@ struct zt_ctx {
   ztg::DevBuf spill;  // traversal stack overflow, sized per launch (grow-only)
   ztg::DevBuf bands;  // per-XCD tile counters of the persistent megakernel
   ztg::DevBuf jobs;   // scene table of a many-scene wave-scheduled launch
+  ztg::DevBuf tile_cost, tile_order;  // longest-first tile order of the wave-scheduled megakernel
   ztg::WfBuffers wf;
   std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };

@@ -155,9 +155,13 @@ struct ManyScenes {
   ZT_D float* output(uint32_t si) const { return jobs[si].out; }
 };
 
-template <bool Q, typename Src>
+// `order` (may be null) lists the tiles in the order they are handed out
+// within each band; the cost probe (kCost) instead counts the rays it starts
+// per tile into `cost`.
+template <bool Q, bool kCost, typename Src>
 ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap,
-                             uint32_t refill, int* spill, uint32_t* band_next, int* err) {
+                             uint32_t refill, int* spill, uint32_t* band_next, const uint32_t* order,
+                             uint32_t* cost, int* err) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, cap, spill);
@@ -194,8 +198,8 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
           if (lane == 0) t = bandStart(band, ntiles) + atomicAdd(&band_next[band], 1u);
           t = __shfl(t, 0);
           if (t < bandStart(band + 1, ntiles)) {
-            tile = t;
-            tile_scene = src.sceneOf(t);
+            tile = order ? order[t] : t;
+            tile_scene = src.sceneOf(tile);
             next_px = 0;
             break;
           }
@@ -254,6 +258,8 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
         tr.start(Query4{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh},
                  sh ? st.sh_tmax : INFINITY);
         tracing = true;
+        if constexpr (kCost)
+          atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], 1u);
         break;
       }
     }
@@ -284,24 +290,59 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   if (overflow) atomicOr(err, 1);
 }
 
-template <int W, bool Q>
+template <int W, bool Q, bool kCost = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
     OneScene src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
-    uint32_t* band_next, int* err) {
-  megakernelWaveBody<Q>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
+    uint32_t* band_next, const uint32_t* order, uint32_t* cost, int* err) {
+  megakernelWaveBody<Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost, err);
 }
 
 // Many scenes in one persistent launch (zt_gpu_render_batch): the same
 // kernel over the concatenated tiles of every scene.
-template <int W, bool Q>
+template <int W, bool Q, bool kCost = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWaveBatch(
     ManyScenes src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
-    uint32_t* band_next, int* err) {
-  megakernelWaveBody<Q>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, err);
+    uint32_t* band_next, const uint32_t* order, uint32_t* cost, int* err) {
+  megakernelWaveBody<Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost, err);
 }
 
 __global__ void clearBands(uint32_t* band_next) {
   if (threadIdx.x < 8) band_next[threadIdx.x] = 0;
+}
+
+__global__ void zeroWords(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Cost bucket of a tile's probe ray count: the position of its leading bit
+// and the two bits below it, so buckets grow in steps of at most 25%.
+ZT_D uint32_t costBucket(uint32_t c) {
+  const uint32_t v = c + 1u;
+  const uint32_t e = 31u - (uint32_t)__clz(v);
+  const uint32_t m = e >= 2 ? (v >> (e - 2)) & 3u : (v << (2 - e)) & 3u;
+  return e * 4u + m;  // < 128
+}
+
+// Longest-first tile order: one workgroup per band sorts the band's tiles by
+// cost bucket, most expensive first (order within a bucket is arbitrary;
+// the image does not depend on it), so a band ends with its cheapest tiles
+// and the last pixels in flight are short.
+__global__ __launch_bounds__(256) void orderTiles(const uint32_t* cost, uint32_t* order, uint32_t ntiles) {
+  __shared__ uint32_t count[128], next[128];
+  const uint32_t band = blockIdx.x, lo = bandStart(band, ntiles), hi = bandStart(band + 1, ntiles);
+  for (uint32_t i = threadIdx.x; i < 128; i += blockDim.x) count[i] = 0;
+  __syncthreads();
+  for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) atomicAdd(&count[costBucket(cost[t])], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = lo;
+    for (int b = 127; b >= 0; --b) {
+      next[b] = run;
+      run += count[b];
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) order[atomicAdd(&next[costBucket(cost[t])], 1u)] = t;
 }
 
 // traceWith for caller-given rays and RNG states (render.zig:61-102).
@@ -465,30 +506,70 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
        (const void*)megakernelWaveBatch<5, false>, (const void*)megakernelWaveBatch<6, false>},
       {(const void*)megakernelWaveBatch<1, true>, (const void*)megakernelWaveBatch<4, true>,
        (const void*)megakernelWaveBatch<5, true>, (const void*)megakernelWaveBatch<6, true>}};
+  static const void* const kProbe[2][2] = {
+      {(const void*)megakernelWave<4, false, true>, (const void*)megakernelWave<4, true, true>},
+      {(const void*)megakernelWaveBatch<4, false, true>, (const void*)megakernelWaveBatch<4, true, true>}};
   const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
+  // Longest-first tile order (tuning tile_order: 0 = when the frame has at
+  // least 16 samples per pixel on average, so the one-sample cost probe adds
+  // at most 1/16; 1 = scan order; 2 = always): a one-sample render counts the
+  // rays each tile starts, orderTiles sorts every band by that count.
+  if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
+  uint64_t pixels = 0, paths = 0;
+  for (const WaveJob& j : jobs) {
+    pixels += (uint64_t)j.pr.width * j.pr.height;
+    paths += (uint64_t)j.pr.width * j.pr.height * j.pr.spp;
+  }
+  const bool ordered = tu.tile_order == 2 || (tu.tile_order == 0 && paths >= 16 * pixels);
+  const void* probe_fn = kProbe[many ? 1 : 0][quant ? 1 : 0];
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
-  ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
+  const uint32_t probe_grid =
+      ordered ? (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(probe_fn, threads, lds)) : 0;
+  ctx->spill.ensure(spillBytes((uint64_t)std::max(grid, probe_grid) * threads));
   ctx->bands.ensure(8 * sizeof(uint32_t));
   uint32_t* bands = ctx->bands.as<uint32_t>();
   int* spill = ctx->spill.as<int>();
   int* err = ctx->errflag.as<int>();
   const uint32_t nt = (uint32_t)ntiles;
-  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
-  // All instantiations take the same arguments; launch the one selected above.
-  if (!many) {
-    OneScene src{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
-    void* args[] = {&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands, &err};
-    ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
-  } else {
-    ctx->jobs.ensure(jobs.size() * sizeof(WaveJob));
-    ZT_HIP(hipMemcpyAsync(ctx->jobs.p, jobs.data(), jobs.size() * sizeof(WaveJob), hipMemcpyHostToDevice,
-                          ctx->stream));
-    ManyScenes src{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
-    void* args[] = {&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands, &err};
-    ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
+  const uint32_t* order = nullptr;
+  uint32_t* no_cost = nullptr;
+  // One scene: everything in kernel arguments. Many: the scene table (and the
+  // probe's copy of it, one sample per pixel) in device memory.
+  std::vector<WaveJob> probe_jobs(jobs);
+  for (WaveJob& j : probe_jobs) j.pr.spp = 1;
+  OneScene one{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
+  OneScene one_probe{jobs[0].view, probe_jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
+  ManyScenes table{}, table_probe{};
+  if (many) {
+    const size_t bytes = jobs.size() * sizeof(WaveJob);
+    ctx->jobs.ensure(2 * bytes);
+    ZT_HIP(hipMemcpyAsync(ctx->jobs.p, jobs.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    ZT_HIP(hipMemcpyAsync((char*)ctx->jobs.p + bytes, probe_jobs.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    table = ManyScenes{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
+    table_probe = ManyScenes{ctx->jobs.as<WaveJob>() + jobs.size(), (uint32_t)jobs.size()};
   }
+  // All instantiations of a form take the same arguments.
+  auto launch = [&](const void* f, uint32_t g, bool probe, const uint32_t* ord, uint32_t* cost) {
+    void* src = many ? (void*)(probe ? &table_probe : &table) : (void*)(probe ? &one_probe : &one);
+    void* args[] = {src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands,
+                    (void*)&ord, (void*)&cost, &err};
+    hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
+    ZT_HIP(hipLaunchKernel(f, dim3(g), dim3(threads), args, lds, ctx->stream));
+    ctx->stats.launches += 2;
+  };
+  if (ordered) {
+    ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
+    ctx->tile_order.ensure((size_t)nt * sizeof(uint32_t));
+    uint32_t* cost = ctx->tile_cost.as<uint32_t>();
+    hipLaunchKernelGGL(zeroWords, dim3((nt + 255) / 256 < 1024 ? (nt + 255) / 256 : 1024), dim3(256), 0, ctx->stream,
+                       cost, nt);
+    launch(probe_fn, probe_grid, true, nullptr, cost);
+    hipLaunchKernelGGL(orderTiles, dim3(8), dim3(256), 0, ctx->stream, cost, ctx->tile_order.as<uint32_t>(), nt);
+    ctx->stats.launches += 2;
+    order = ctx->tile_order.as<uint32_t>();
+  }
+  launch(fn, grid, false, order, no_cost);
   ZT_HIP(hipGetLastError());
-  ctx->stats.launches += 2;
 }
 
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds) {

@@ -49,12 +49,14 @@ class Tuning:
     schedule: int = 0
     refill: int = 0
     node_bytes: int = 0
+    tile_order: int = 0
 
     def record(self) -> np.ndarray:
         return N.record(N.TUNING, method=self.method, precision=self.precision, tile_w=self.tile_w,
                         tile_h=self.tile_h, block_size=self.block_size, queue_paths=self.queue_paths,
                         variant=self.variant, stack_entries=self.stack_entries, bvh_width=self.bvh_width,
-                        schedule=self.schedule, refill=self.refill, node_bytes=self.node_bytes)
+                        schedule=self.schedule, refill=self.refill, node_bytes=self.node_bytes,
+                        tile_order=self.tile_order)
 
 
 def render(scene: Scene, opts: Options, tuning: Tuning | None = None) -> np.ndarray:
