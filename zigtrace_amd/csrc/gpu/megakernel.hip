@@ -513,7 +513,8 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   // Longest-first tile order (tuning tile_order: 0 = when the frame has at
   // least 16 samples per pixel on average, so the one-sample cost probe adds
   // at most 1/16; 1 = scan order; 2 = always): a one-sample render counts the
-  // rays each tile starts, orderTiles sorts every band by that count.
+  // rays each tile starts, orderTiles sorts every band by that count. (A
+  // probe of one pixel per 2x2 block orders worse than it saves.)
   if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
   uint64_t pixels = 0, paths = 0;
   for (const WaveJob& j : jobs) {
