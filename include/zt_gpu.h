@@ -86,8 +86,8 @@ typedef struct zt_gpu_tuning {
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,
                              2 = most expensive first (a one-sample probe render counts the rays of every tile,
-                             then the tiles of each band are sorted by that count), 0 = 2 when the frame (or
-                             batch) has at least 16 samples per pixel on average, else 1 */
+                             then the tiles of each band are sorted by that count), 0 = 2 for a single scene
+                             with at least 16 samples per pixel, else 1 */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

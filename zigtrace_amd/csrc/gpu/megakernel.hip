@@ -510,18 +510,20 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
       {(const void*)megakernelWave<4, false, true>, (const void*)megakernelWave<4, true, true>},
       {(const void*)megakernelWaveBatch<4, false, true>, (const void*)megakernelWaveBatch<4, true, true>}};
   const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
-  // Longest-first tile order (tuning tile_order: 0 = when the frame has at
-  // least 16 samples per pixel on average, so the one-sample cost probe adds
-  // at most 1/16; 1 = scan order; 2 = always): a one-sample render counts the
-  // rays each tile starts, orderTiles sorts every band by that count. (A
-  // probe of one pixel per 2x2 block orders worse than it saves.)
+  // Longest-first tile order (tuning tile_order: 0 = for one scene with at
+  // least 16 samples per pixel, so the one-sample cost probe adds at most
+  // 1/16; 1 = scan order; 2 = always): a one-sample render counts the rays
+  // each tile starts, orderTiles sorts every band by that count. (A probe of
+  // one pixel per 2x2 block orders worse than it saves; across the scenes of
+  // a batch the order mixes scenes within each wave and loses more than the
+  // shorter tail gains.)
   if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
   uint64_t pixels = 0, paths = 0;
   for (const WaveJob& j : jobs) {
     pixels += (uint64_t)j.pr.width * j.pr.height;
     paths += (uint64_t)j.pr.width * j.pr.height * j.pr.spp;
   }
-  const bool ordered = tu.tile_order == 2 || (tu.tile_order == 0 && paths >= 16 * pixels);
+  const bool ordered = tu.tile_order == 2 || (tu.tile_order == 0 && !many && paths >= 16 * pixels);
   const void* probe_fn = kProbe[many ? 1 : 0][quant ? 1 : 0];
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
   const uint32_t probe_grid =
