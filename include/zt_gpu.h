@@ -85,9 +85,9 @@ typedef struct zt_gpu_tuning {
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,
-                             2 = most expensive first (a one-sample probe render counts the rays of every tile,
-                             then the tiles of each band are sorted by that count), 0 = 2 for a single scene
-                             with at least 16 samples per pixel, else 1 */
+                             2 = most expensive first (a one-sample probe render without next-event rays counts
+                             the extension rays of every tile, then the tiles of each band are sorted by that
+                             count), 0 = 2 for a single scene with at least 16 samples per pixel, else 1 */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

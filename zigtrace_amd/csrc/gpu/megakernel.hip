@@ -512,11 +512,12 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
   // Longest-first tile order (tuning tile_order: 0 = for one scene with at
   // least 16 samples per pixel, so the one-sample cost probe adds at most
-  // 1/16; 1 = scan order; 2 = always): a one-sample render counts the rays
-  // each tile starts, orderTiles sorts every band by that count. (A probe of
-  // one pixel per 2x2 block orders worse than it saves; across the scenes of
-  // a batch the order mixes scenes within each wave and loses more than the
-  // shorter tail gains.)
+  // 1/16; 1 = scan order; 2 = always): a one-sample render without shadow
+  // rays counts the rays each tile starts, orderTiles sorts every band by
+  // that count. (A probe of one pixel per 2x2 block, or with paths cut at
+  // depth 6, orders worse than it saves; across the scenes of a batch the
+  // order mixes scenes within each wave and loses more than the shorter
+  // tail gains.)
   if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
   uint64_t pixels = 0, paths = 0;
   for (const WaveJob& j : jobs) {
@@ -539,7 +540,10 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   // One scene: everything in kernel arguments. Many: the scene table (and the
   // probe's copy of it, one sample per pixel) in device memory.
   std::vector<WaveJob> probe_jobs(jobs);
-  for (WaveJob& j : probe_jobs) j.pr.spp = 1;
+  for (WaveJob& j : probe_jobs) {  // one sample; extension rays only (they rank tiles as well)
+    j.pr.spp = 1;
+    j.pr.use_nee = 0;
+  }
   OneScene one{jobs[0].view, jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
   OneScene one_probe{jobs[0].view, probe_jobs[0].pr, jobs[0].out, jobs[0].tiles_x};
   ManyScenes table{}, table_probe{};
