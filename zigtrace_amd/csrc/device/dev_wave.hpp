@@ -1,6 +1,7 @@
 // Wave-level pieces of the kernels that walk the four-wide BVH with one ray
 // per lane under wave-uniform passes (the wave-scheduled megakernel). `Src`
-// gives the scene view of a lane's scene index: `src.view(si)`. The
+// gives the scene view of a lane's scene index, `src.view(si)`, and says
+// whether every lane renders the same scene (`Src::kSingle`). The
 // workgroup must be exactly one wave: the leaf pass orders its LDS traffic
 // with __syncthreads, which is then a wave-local barrier that every pass
 // reaches together.
@@ -62,7 +63,7 @@ ZT_D bool leafPassWave(const Src& src, uint32_t si, Trav4& tr, const Query4& q, 
     const int self = __shfl(self2, own) >> 1;
     const int excl = __shfl(q.excl, own);
     const int k = __shfl(base, own) + (int)w;
-    const uint32_t osi = __shfl(si, own);
+    const uint32_t osi = Src::kSingle ? 0u : __shfl(si, own);
     float t;
     if (live && primHit<float>(src.view(osi), k, o, d, q.tmin, tmax, self, excl, &t))
       atomicMin(&keys[own], ((unsigned long long)__float_as_uint(t) << 32) | (uint32_t)k);

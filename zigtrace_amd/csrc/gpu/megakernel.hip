@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
 // one index space. One scene: everything in kernel arguments. Many scenes:
 // a table in device memory, each lane reading its own pixel's scene.
 struct OneScene {
+  static constexpr bool kSingle = true;
   SceneView<float> sc;
   Params pr;
   float* out;
@@ -137,6 +138,7 @@ struct OneScene {
 };
 
 struct ManyScenes {
+  static constexpr bool kSingle = false;
   const WaveJob* jobs;
   uint32_t n;
   ZT_D uint32_t sceneOf(uint32_t tile) const {  // last job whose first tile <= tile
