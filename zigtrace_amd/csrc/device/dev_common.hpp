@@ -161,6 +161,10 @@ struct __align__(128) DNode4 {
   int4 _pad;
 };
 static_assert(sizeof(DNode4) == 128, "one four-wide node per 128-byte line");
+// Most four-wide nodes a scene may have: node byte offsets (index * 128)
+// are 32-bit, so traversal addresses a node as base + one 32-bit offset.
+static constexpr uint32_t kMaxNodes4 = 1u << 25;
+static_assert((uint64_t)kMaxNodes4 * sizeof(DNode4) <= (1ull << 32), "32-bit node offsets");
 
 // The same four-wide node in 64 bytes: child boxes as 8-bit offsets from a
 // per-node origin in steps of a power of two per axis. A bound decodes as

@@ -112,21 +112,29 @@ ZT_D bool planeHit(const Plane<R>& pl, V3<R> o, V3<R> d, R tmin, R tmax, R* tout
 // current closest hit; a NaN slab (0*inf) leaves the interval unchanged,
 // which fmax/fmin reproduce. The fp32 path widens the far distance by
 // 1 + 2*gamma(3) so rounding never loses a box the exact test would hit.
+// slabNear: the same test given each axis's near plane (n*) and far plane
+// (f*) along the ray.
 template <typename R>
-ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1, V3<R> o, V3<R> inv, bool nx,
-               bool ny, bool nz, R tmin, R tmax, R* tout) {
-  const R tx0 = (R(nx ? bx1 : bx0) - o.x) * inv.x;
-  const R tx1 = (R(nx ? bx0 : bx1) - o.x) * inv.x;
-  const R ty0 = (R(ny ? by1 : by0) - o.y) * inv.y;
-  const R ty1 = (R(ny ? by0 : by1) - o.y) * inv.y;
-  const R tz0 = (R(nz ? bz1 : bz0) - o.z) * inv.z;
-  const R tz1 = (R(nz ? bz0 : bz1) - o.z) * inv.z;
+ZT_D bool slabNear(float nx, float ny, float nz, float fx, float fy, float fz, V3<R> o, V3<R> inv, R tmin, R tmax,
+                   R* tout) {
+  const R tx0 = (R(nx) - o.x) * inv.x;
+  const R tx1 = (R(fx) - o.x) * inv.x;
+  const R ty0 = (R(ny) - o.y) * inv.y;
+  const R ty1 = (R(fy) - o.y) * inv.y;
+  const R tz0 = (R(nz) - o.z) * inv.z;
+  const R tz1 = (R(fz) - o.z) * inv.z;
   const R lo = fmax(fmax(tmin, tx0), fmax(ty0, tz0));
   R far = fmin(tx1, fmin(ty1, tz1));
   if constexpr (!K<R>::exact) far = far * (1.0f + 3.0f * 0x1.0p-23f);
   const R hi = fmin(tmax, far);
   *tout = lo;
   return !(hi < lo);
+}
+template <typename R>
+ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1, V3<R> o, V3<R> inv, bool nx,
+               bool ny, bool nz, R tmin, R tmax, R* tout) {
+  return slabNear<R>(nx ? bx1 : bx0, ny ? by1 : by0, nz ? bz1 : bz0, nx ? bx0 : bx1, ny ? by0 : by1, nz ? bz0 : bz1,
+                     o, inv, tmin, tmax, tout);
 }
 
 // Per-lane traversal stack: the top `cap` entries live in LDS (laid out
@@ -156,10 +164,14 @@ struct LaneStack {
 ZT_HD size_t stackBytes(int cap, int threads) { return (size_t)2 * cap * threads * sizeof(int); }
 // Global spill bytes for a launch of `threads` threads.
 ZT_HD size_t spillBytes(uint64_t threads) { return (size_t)2 * kSpill * threads * sizeof(int); }
+// kLanes: the workgroup size when the kernel fixes it (then the entry
+// addresses take a shift instead of a 32-bit multiply), else 0.
+template <int kLanes = 0>
 ZT_D LaneStack laneStack(int* lds, int cap, int* spill) {
-  const uint32_t threads = gridDim.x * blockDim.x;
-  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  return LaneStack{lds + threadIdx.x, (float*)(lds + cap * blockDim.x) + threadIdx.x, (int)blockDim.x, cap,
+  const uint32_t lanes = kLanes ? (uint32_t)kLanes : blockDim.x;
+  const uint32_t threads = gridDim.x * lanes;
+  const uint32_t gtid = blockIdx.x * lanes + threadIdx.x;
+  return LaneStack{lds + threadIdx.x, (float*)(lds + cap * lanes) + threadIdx.x, (int)lanes, cap,
                    spill + gtid, (float*)(spill + (size_t)kSpill * threads) + gtid, threads};
 }
 
@@ -404,15 +416,17 @@ struct Trav4 {
       nsp += half;
       sp -= half;
     }
-    stk.code[sp * stk.stride] = m == 3 ? c3 : (m == 2 ? c2 : c1);
-    stk.tval[sp * stk.stride] = m == 3 ? k3 : (m == 2 ? k2 : k1);
+    // c1 on top, then c2, then c3: each written to its slot, no selects.
+    const int top = sp + m - 1;
+    stk.code[top * stk.stride] = c1;
+    stk.tval[top * stk.stride] = k1;
     if (m >= 2) {
-      stk.code[(sp + 1) * stk.stride] = m == 3 ? c2 : c1;
-      stk.tval[(sp + 1) * stk.stride] = m == 3 ? k2 : k1;
+      stk.code[(top - 1) * stk.stride] = c2;
+      stk.tval[(top - 1) * stk.stride] = k2;
     }
     if (m == 3) {
-      stk.code[(sp + 2) * stk.stride] = c1;
-      stk.tval[(sp + 2) * stk.stride] = k1;
+      stk.code[(top - 2) * stk.stride] = c3;
+      stk.tval[(top - 2) * stk.stride] = k3;
     }
     sp += m;
   }
@@ -437,9 +451,12 @@ struct Trav4 {
       if (stk.tval[sp * stk.stride] <= closest) cur = stk.code[sp * stk.stride];
     }
     if (cur < 0) return false;
-    float4 lx, ly, lz, hx, hy, hz;
+    float k0, k1, k2, k3;
+    bool h0, h1, h2, h3;
     int c0, c1, c2, c3;
+    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     if constexpr (Q) {
+      float4 lx, ly, lz, hx, hy, hz;
       const uint4* p = (const uint4*)(sc.nodes4q + cur);
       const uint4 a = p[0], b = p[1], c = p[2];
       const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
@@ -457,25 +474,30 @@ struct Trav4 {
       const uint2 d = *(const uint2*)(p + 3);
       lz = decodeBounds(d.x, sz, oz);
       hz = decodeBounds(d.y, sz, oz);
+      h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, q.o, inv, nx, ny, nz, q.tmin, closest, &k0);
+      h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, q.o, inv, nx, ny, nz, q.tmin, closest, &k1);
+      h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, q.o, inv, nx, ny, nz, q.tmin, closest, &k2);
+      h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, q.o, inv, nx, ny, nz, q.tmin, closest, &k3);
     } else {
-      const DNode4* n = sc.nodes4 + cur;
-      lx = n->lox;
-      ly = n->loy;
-      lz = n->loz;
-      hx = n->hix;
-      hy = n->hiy;
-      hz = n->hiz;
-      c0 = n->child.x;
-      c1 = n->child.y;
-      c2 = n->child.z;
-      c3 = n->child.w;
+      // Each axis's near and far rows (lo, or hi when the direction is
+      // negative) picked by address -- byte offsets lo 0/16/32, hi 48/64/80
+      // -- instead of by a select per child and bound.
+      const char* base = (const char*)sc.nodes4;
+      const uint32_t at = (uint32_t)cur << 7;
+      const uint32_t ox = nx ? 48u : 0u, oy = ny ? 64u : 16u, oz = nz ? 80u : 32u;
+      const float4 ax = *(const float4*)(base + (at | ox)), fx = *(const float4*)(base + (at | (ox ^ 48u)));
+      const float4 ay = *(const float4*)(base + (at | oy)), fy = *(const float4*)(base + (at | (oy ^ 80u)));
+      const float4 az = *(const float4*)(base + (at | oz)), fz = *(const float4*)(base + (at | (oz ^ 112u)));
+      const int4 ch = *(const int4*)(base + (at | 96u));
+      c0 = ch.x;
+      c1 = ch.y;
+      c2 = ch.z;
+      c3 = ch.w;
+      h0 = slabNear<float>(ax.x, ay.x, az.x, fx.x, fy.x, fz.x, q.o, inv, q.tmin, closest, &k0);
+      h1 = slabNear<float>(ax.y, ay.y, az.y, fx.y, fy.y, fz.y, q.o, inv, q.tmin, closest, &k1);
+      h2 = slabNear<float>(ax.z, ay.z, az.z, fx.z, fy.z, fz.z, q.o, inv, q.tmin, closest, &k2);
+      h3 = slabNear<float>(ax.w, ay.w, az.w, fx.w, fy.w, fz.w, q.o, inv, q.tmin, closest, &k3);
     }
-    float k0, k1, k2, k3;
-    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    const bool h0 = slab<float>(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, q.o, inv, nx, ny, nz, q.tmin, closest, &k0);
-    const bool h1 = slab<float>(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, q.o, inv, nx, ny, nz, q.tmin, closest, &k1);
-    const bool h2 = slab<float>(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, q.o, inv, nx, ny, nz, q.tmin, closest, &k2);
-    const bool h3 = slab<float>(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, q.o, inv, nx, ny, nz, q.tmin, closest, &k3);
     // Misses (and empty slots, whose boxes are inverted) sort last.
     const bool v0 = h0 && c0 != kNoChild, v1 = h1 && c1 != kNoChild, v2 = h2 && c2 != kNoChild,
                v3 = h3 && c3 != kNoChild;

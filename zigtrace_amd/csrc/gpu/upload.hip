@@ -555,7 +555,7 @@ int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id)
         Collapse4 c4{tb.out, {}, 0};
         c4.node(0, 0);
         s->tree_depth4 = c4.max_depth;
-        if (3 * c4.max_depth + 1 <= 8 + kSpill) {
+        if (3 * c4.max_depth + 1 <= 8 + kSpill && c4.out.size() <= kMaxNodes4) {
           s->num_nodes4 = (int)c4.out.size();
           s->nodes4.upload(c4.out, ctx->stream);
           std::vector<DNode4Q> q(c4.out.size());
