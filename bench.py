@@ -81,7 +81,7 @@ def cpu_baseline(workload, target_seconds: float = 15.0):
     rays = int(ctr["extend_rays"]) + int(ctr["shadow_rays"])
     return {
         "value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-        "sample": f"one pixel in {stride} of the {o.width}x{o.height} frame, all {o.spp} spp ({paths} paths, "
+        "sample": f"{'every pixel' if stride == 1 else f'one pixel in {stride}'} of the {o.width}x{o.height} frame, all {o.spp} spp ({paths} paths, "
                   f"{dt:.1f}s), C++ f64 restatement of the reference CPU path",
     }, ctr, rays / max(paths, 1)
 
