@@ -260,6 +260,28 @@ struct SceneView {
   V3<R> bg_a, bg_b;
 };
 
+// 16-byte loads from device memory through the global address space. The
+// batched kernels read their scene's arrays through pointers that are
+// themselves loaded from the scene table, which the compiler cannot prove
+// global; it would emit flat loads, which also count against (and wait
+// with) the wave's LDS traffic. Every pointer passed here addresses device
+// memory.
+typedef float GF4 __attribute__((ext_vector_type(4)));
+typedef int GI4 __attribute__((ext_vector_type(4)));
+ZT_D GF4 loadGF4(const void* base, uint64_t byte_off) {
+  return *(const __attribute__((address_space(1))) GF4*)((const __attribute__((address_space(1))) char*)base +
+                                                          byte_off);
+}
+ZT_D V3<float> loadGF3(const void* base, uint64_t byte_off) {
+  const __attribute__((address_space(1))) float* p =
+      (const __attribute__((address_space(1))) float*)((const __attribute__((address_space(1))) char*)base + byte_off);
+  return V3<float>{p[0], p[1], p[2]};
+}
+ZT_D GI4 loadGI4(const void* base, uint64_t byte_off) {
+  return *(const __attribute__((address_space(1))) GI4*)((const __attribute__((address_space(1))) char*)base +
+                                                          byte_off);
+}
+
 // Decoded primitive for the intersection routines.
 template <typename R>
 struct Prim {
@@ -268,15 +290,14 @@ struct Prim {
 };
 template <typename R> ZT_D Prim<R> loadPrim(const SceneView<R>& sc, int k);
 template <> ZT_D Prim<float> loadPrim<float>(const SceneView<float>& sc, int k) {
-  const PrimF* p = (const PrimF*)sc.prims + k;
-  const float4 p0 = p->p0, p1 = p->p1;
+  const uint64_t at = (uint64_t)k * sizeof(PrimF);
+  const GF4 p0 = loadGF4(sc.prims, at);
   Prim<float> r;
   r.a = mk(p0.x, p0.y, p0.z);
   r.meta = __float_as_uint(p0.w);
-  r.b = mk(p1.x, p1.y, p1.z);
+  r.b = loadGF3(sc.prims, at + 16);
   if (metaKind(r.meta) == kKindTriangle) {
-    const float4 p2 = p->p2;
-    r.c = mk(p2.x, p2.y, p2.z);
+    r.c = loadGF3(sc.prims, at + 32);
   } else {
     r.c = mk(0.f, 0.f, 0.f);
   }

@@ -482,13 +482,13 @@ struct Trav4 {
       // Each axis's near and far rows (lo, or hi when the direction is
       // negative) picked by address -- byte offsets lo 0/16/32, hi 48/64/80
       // -- instead of by a select per child and bound.
-      const char* base = (const char*)sc.nodes4;
+      const void* base = sc.nodes4;
       const uint32_t at = (uint32_t)cur << 7;
       const uint32_t ox = nx ? 48u : 0u, oy = ny ? 64u : 16u, oz = nz ? 80u : 32u;
-      const float4 ax = *(const float4*)(base + (at | ox)), fx = *(const float4*)(base + (at | (ox ^ 48u)));
-      const float4 ay = *(const float4*)(base + (at | oy)), fy = *(const float4*)(base + (at | (oy ^ 80u)));
-      const float4 az = *(const float4*)(base + (at | oz)), fz = *(const float4*)(base + (at | (oz ^ 112u)));
-      const int4 ch = *(const int4*)(base + (at | 96u));
+      const GF4 ax = loadGF4(base, at | ox), fx = loadGF4(base, at | (ox ^ 48u));
+      const GF4 ay = loadGF4(base, at | oy), fy = loadGF4(base, at | (oy ^ 80u));
+      const GF4 az = loadGF4(base, at | oz), fz = loadGF4(base, at | (oz ^ 112u));
+      const GI4 ch = loadGI4(base, at | 96u);
       c0 = ch.x;
       c1 = ch.y;
       c2 = ch.z;
