@@ -120,7 +120,10 @@ int zt_gpu_copy_to_host(zt_ctx* ctx, void* host, const void* device, uint64_t by
 /* Scenes ----------------------------------------------------------------- */
 /* Copies a built scene to the device in the GPU layout (64-byte two-child
  * BVH nodes, leaf-ordered primitives, fp32 and on request fp64 copies).
- * Validates every index; returns ZT_ERR_BAD_SCENE on inconsistency. */
+ * Validates every index; returns ZT_ERR_BAD_SCENE on inconsistency. Also
+ * builds the four-wide tree (128-byte nodes) unless it would be deeper than
+ * the traversal stack allows or hold more than 2^25 nodes; such a scene is
+ * rendered over the two-child tree. */
 int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* view, uint32_t* scene_id);
 int zt_gpu_scene_release(zt_ctx* ctx, uint32_t scene_id);
 

@@ -166,7 +166,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
                              uint32_t* cost, int* err) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
   extern __shared__ int lds[];
-  const LaneStack stk = laneStack<64>(lds, cap, spill);
+  const LaneStack stk = laneStack<64>(lds, cap, spill);  // workgroups of one wave (64 threads)
   unsigned long long* leaf_keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
   uint8_t* leaf_owner = (uint8_t*)(leaf_keys + 64);
   const uint32_t lane = threadIdx.x;
