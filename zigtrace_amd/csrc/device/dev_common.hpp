@@ -185,7 +185,8 @@ static constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
 ZT_HD int leafCode(uint32_t first, uint32_t count) { return -1 - (int)((first << 4) | count); }
 
 // Primitive records in leaf order. kind in bits 30-31 of meta, material in 0-29.
-//   fp32: p0 = (v0 or centre, meta bits), p1 = (e1 or (radius,0,0)), p2 = e2
+//   fp32: p0 = (v0 or centre, meta bits), p1 = (e1 or (radius,0,0)),
+//         p2 = (e2 or (0,0,0), meta bits again)
 //   fp64: a[0..2] v0/centre, a[3..5] e1/(radius,..), a[6..8] e2
 struct __align__(16) PrimF {
   float4 p0, p1, p2;
@@ -290,17 +291,16 @@ struct Prim {
 };
 template <typename R> ZT_D Prim<R> loadPrim(const SceneView<R>& sc, int k);
 template <> ZT_D Prim<float> loadPrim<float>(const SceneView<float>& sc, int k) {
+  // The kind is read from the third vector, which every caller then needs
+  // before it branches on the kind: the three loads go out together (with
+  // the kind taken from the first, a triangle's e2 waited for it to return).
   const uint64_t at = (uint64_t)k * sizeof(PrimF);
-  const GF4 p0 = loadGF4(sc.prims, at);
+  const GF4 p2 = loadGF4(sc.prims, at + 32);
   Prim<float> r;
-  r.a = mk(p0.x, p0.y, p0.z);
-  r.meta = __float_as_uint(p0.w);
+  r.a = loadGF3(sc.prims, at);
   r.b = loadGF3(sc.prims, at + 16);
-  if (metaKind(r.meta) == kKindTriangle) {
-    r.c = loadGF3(sc.prims, at + 32);
-  } else {
-    r.c = mk(0.f, 0.f, 0.f);
-  }
+  r.c = mk(p2.x, p2.y, p2.z);
+  r.meta = __float_as_uint(p2.w);
   return r;
 }
 template <> ZT_D Prim<double> loadPrim<double>(const SceneView<double>& sc, int k) {

@@ -592,7 +592,7 @@ int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id)
       std::memcpy(&meta_f, &meta, 4);
       f.p0 = make_float4((float)d.a[0], (float)d.a[1], (float)d.a[2], meta_f);
       f.p1 = make_float4((float)d.a[3], (float)d.a[4], (float)d.a[5], 0.f);
-      f.p2 = make_float4((float)d.a[6], (float)d.a[7], (float)d.a[8], 0.f);
+      f.p2 = make_float4((float)d.a[6], (float)d.a[7], (float)d.a[8], meta_f);
       pf[k] = f;
     }
     s->primsF.upload(pf, ctx->stream);
