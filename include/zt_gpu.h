@@ -87,7 +87,8 @@ typedef struct zt_gpu_tuning {
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,
                              2 = most expensive first (a one-sample probe render without next-event rays counts
                              the extension rays of every tile, then the tiles of each band are sorted by that
-                             count), 0 = 2 for a single scene with at least 16 samples per pixel, else 1 */
+                             count), 0 = 2 for a single scene with at least 16 samples per pixel and more
+                             tiles than the GPU holds waves at once, else 1 */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

@@ -514,7 +514,9 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
   // Longest-first tile order (tuning tile_order: 0 = for one scene with at
   // least 16 samples per pixel, so the one-sample cost probe adds at most
-  // 1/16; 1 = scan order; 2 = always): a one-sample render without shadow
+  // 1/16, and more tiles than resident waves -- when every tile starts at
+  // once their order cannot matter; 1 = scan order; 2 = always): a
+  // one-sample render without shadow
   // rays counts the rays each tile starts, orderTiles sorts every band by
   // that count. (A probe of one pixel per 2x2 block, or with paths cut at
   // depth 6, orders worse than it saves; across the scenes of a batch the
@@ -526,9 +528,10 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
     pixels += (uint64_t)j.pr.width * j.pr.height;
     paths += (uint64_t)j.pr.width * j.pr.height * j.pr.spp;
   }
-  const bool ordered = tu.tile_order == 2 || (tu.tile_order == 0 && !many && paths >= 16 * pixels);
-  const void* probe_fn = kProbe[many ? 1 : 0][quant ? 1 : 0];
   const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
+  const bool ordered =
+      tu.tile_order == 2 || (tu.tile_order == 0 && !many && paths >= 16 * pixels && ntiles > grid);
+  const void* probe_fn = kProbe[many ? 1 : 0][quant ? 1 : 0];
   const uint32_t probe_grid =
       ordered ? (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(probe_fn, threads, lds)) : 0;
   ctx->spill.ensure(spillBytes((uint64_t)std::max(grid, probe_grid) * threads));
