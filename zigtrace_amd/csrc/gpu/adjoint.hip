@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void adjointKernel(SceneView<R> sc, Params pr,
                                                      uint32_t ntiles, int* spill, int* err) {
   extern __shared__ int lds[];
   const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack(lds, kLds, spill);
+  const LaneStack stk = laneStack<64>(lds, kLds, spill);  // workgroups of one wave (launchAdjoint)
   R* lg = (R*)(lds + 2 * kLds * blockDim.x);
   for (int i = (int)tid; i < nparams; i += (int)blockDim.x) lg[i] = R(0);
   __syncthreads();
