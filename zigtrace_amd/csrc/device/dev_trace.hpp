@@ -187,19 +187,13 @@ ZT_D float entryT(R t) {
 // Primitive k against a ray whose interval is (tmin, tmax); the fp32 path
 // skips the shadow ray's target and treats the ray's own primitive by
 // identity (a sphere then yields only the far end of its chord).
-// primTest: the same for primitive k already loaded as p.
-template <typename R>
-ZT_D bool primTest(const Prim<R>& p, int k, V3<R> o, V3<R> d, R tmin, R tmax, int self, int excl, R* t) {
-  constexpr bool kIdentity = !K<R>::exact;
-  if (kIdentity && k == excl) return false;
-  if (metaKind(p.meta) == kKindTriangle) return (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, tmax, t);
-  return (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, tmax, t) : sphereHit<R>(p, o, d, tmin, tmax, t);
-}
 template <typename R>
 ZT_D bool primHit(const SceneView<R>& sc, int k, V3<R> o, V3<R> d, R tmin, R tmax, int self, int excl, R* t) {
   constexpr bool kIdentity = !K<R>::exact;
   if (kIdentity && k == excl) return false;
-  return primTest<R>(loadPrim<R>(sc, k), k, o, d, tmin, tmax, self, excl, t);
+  const Prim<R> p = loadPrim<R>(sc, k);
+  if (metaKind(p.meta) == kKindTriangle) return (kIdentity && k == self) ? false : triangleHit<R>(p, o, d, tmin, tmax, t);
+  return (kIdentity && k == self) ? selfSphereHit<R>(p, o, d, tmin, tmax, t) : sphereHit<R>(p, o, d, tmin, tmax, t);
 }
 
 template <typename R>
