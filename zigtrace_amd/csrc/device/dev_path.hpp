@@ -130,7 +130,10 @@ struct LightSample {
 };
 
 // Receives the contributions a path adds to its radiance, for the adjoint
-// (path replay). Rendering uses NoObserver, whose empty hooks vanish.
+// (path replay). Rendering uses NoObserver (with a null pointer), whose hooks
+// are compiled out. The hooks are selected at compile time, not by testing
+// the pointer: comparing a lane-private observer's address with null keeps
+// the compiler from holding the observer in registers.
 struct NoObserver {
   template <typename R>
   ZT_D void miss(V3<R>) {}
@@ -141,6 +144,8 @@ struct NoObserver {
   ZT_D void neePending(V3<R>, V3<R>, int) {}
   ZT_D void neeCommitted() {}
 };
+template <typename Obs>
+constexpr bool kObserved = !std::is_same<Obs, NoObserver>::value;
 
 // Lights.pick + Lights.sample (light.zig:106-166). The pick is done on the
 // f64 CDF with the exact f64 uniform in both precisions, so it selects the
@@ -294,7 +299,7 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
   if (h.prim < 0) {
     const V3<R> x = mul(st.throughput, background(sc, st.rd));
     st.radiance = add(st.radiance, x);
-    if (obs) obs->miss(x);
+    if constexpr (kObserved<Obs>) obs->miss(x);
     return kFinish;
   }
   const Surface<R> s = surfaceAt(sc, st.ro, st.rd, h);
@@ -305,7 +310,7 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
     const R w = st.prev_pdf > 0 ? powerWeight(st.prev_pdf, pdfHit(sc, st.ro, s.p, h.prim)) : R(1);
     const V3<R> x = scale(mul(st.throughput, m.color), w);
     st.radiance = add(st.radiance, x);
-    if (obs) obs->emission(x, scale(st.throughput, w), s.material);
+    if constexpr (kObserved<Obs>) obs->emission(x, scale(st.throughput, w), s.material);
   }
   bool pending = false;
   const bool lambertian = m.kind == ZT_LAMBERTIAN;
@@ -321,7 +326,8 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
         st.sh_tmax = ls.distance * R(0.999999);
         st.sh_excl = ls.prim;
         pending = true;
-        if (obs) obs->neePending(st.sh_contrib, scale(mul(st.throughput, m.color), sc_), ls.material);
+        if constexpr (kObserved<Obs>)
+          obs->neePending(st.sh_contrib, scale(mul(st.throughput, m.color), sc_), ls.material);
       }
     }
   }
@@ -331,7 +337,8 @@ ZT_D int shade(const SceneView<R>& sc, const Params& pr, HitRec<R> h, Rng& rng, 
   st.self = h.prim;
   bool alive = scattered;
   if (scattered) {
-    if (obs && (m.kind == ZT_LAMBERTIAN || m.kind == ZT_METAL)) obs->scattered(s.material);
+    if constexpr (kObserved<Obs>)
+      if (m.kind == ZT_LAMBERTIAN || m.kind == ZT_METAL) obs->scattered(s.material);
     st.throughput = mul(st.throughput, att);
     st.prev_pdf = (pr.use_nee && lambertian) ? fmax(dot(dir, s.n), R(0)) / K<R>::pi : R(0);
     st.rd = dir;
@@ -354,7 +361,7 @@ template <typename R, typename Obs = NoObserver>
 ZT_D int afterShadow(HitRec<R> h, Path<R>& st, Obs* obs = nullptr) {
   if (h.prim < 0) {
     st.radiance = add(st.radiance, st.sh_contrib);
-    if (obs) obs->neeCommitted();
+    if constexpr (kObserved<Obs>) obs->neeCommitted();
   }
   return st.terminated ? kFinish : kExtend;
 }

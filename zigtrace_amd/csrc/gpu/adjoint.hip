@@ -155,7 +155,7 @@ struct AdjointObserver {
 };
 
 template <typename R, int K>
-__global__ __launch_bounds__(256) void adjointKernel(SceneView<R> sc, Params pr, const float* __restrict__ dimg,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 8))) void adjointKernel(SceneView<R> sc, Params pr, const float* __restrict__ dimg,
                                                      const int* __restrict__ offsets, int nparams,
                                                      R* __restrict__ grad, uint32_t tile_w, uint32_t tiles_x,
                                                      uint32_t ntiles, int* spill, int* err) {
