@@ -57,7 +57,7 @@ def algorithmic_bytes_per_path(ctr, spp: int) -> float:
     return b / paths + 12.0 / spp
 
 
-def cpu_baseline(workload, target_seconds: float = 15.0):
+def cpu_baseline(workload, target_seconds: float = 20.0):
     """The oracle on a strided pixel sample of the same frame: (record, counters, rays per path)."""
     from oracle import oracle as O
 
@@ -72,7 +72,7 @@ def cpu_baseline(workload, target_seconds: float = 15.0):
         t = time.time()
         _, ctr = ref.render(o.width, o.height, o.spp, o.max_depth, threads=threads, seed=o.seed, pixel_stride=stride)
         dt = time.time() - t
-        if dt > target_seconds / 3 or stride == 1:
+        if dt > target_seconds / 2 or stride == 1:  # keep a sample of at least ~10 s
             break
         rate = int(ctr["paths"]) / max(dt, 1e-9)
         want_paths = rate * target_seconds
