@@ -10,7 +10,8 @@ before the timed region.
 Reported beside `value` (which is the fastest method's whole-frame rate):
   * per-method Mpaths/s (plain megakernel, wavefront), each with the launch
     configuration the autotuner picked,
-  * Mrays/s (extension + shadow rays, counted by the oracle on a sample),
+  * Mrays/s (extension + shadow rays, counted on the device during the timed
+    renders),
   * the roofline of the megakernel: algorithmic bytes per launch (BVH node
     and primitive fetches counted by the oracle on a sample of the same
     frame, 64 B/node, 48 B/triangle, 32 B/sphere, 16 B/plane test, plus the
@@ -22,7 +23,14 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
   * adjoint: forward + path-replay backward iterations/s of Adam inverse
     rendering on the 256x256, 32 spp differentiable box,
   * the CPU baseline: the oracle (C++ f64 restatement of the reference CPU
-    path, tiles + atomic counter, all threads) on a strided pixel sample.
+    path, tiles + atomic counter, all hardware threads) on a strided pixel
+    sample of the same frame and seed, and the image error of the GPU frame
+    against those oracle pixels (relative RMSE, fraction of pixels moved by a
+    diverged path, extension/shadow ray counts of both),
+  * the fp64 verification path's Mpaths/s on the same frame (the reference's
+    precision),
+  * the two smaller configurations (the shipped Cornell box at 400x400, 64 spp;
+    the sphere field at 600x340, 16 spp), GPU beside the oracle.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method auto|megakernel|wavefront]
                        [--no-cpu] [--no-extra] [--no-tune] [--spp S]
@@ -58,19 +66,22 @@ def algorithmic_bytes_per_path(ctr, spp: int) -> float:
 
 
 def cpu_baseline(workload, target_seconds: float = 20.0):
-    """The oracle on a strided pixel sample of the same frame: (record, counters, rays per path)."""
+    """The oracle on a strided pixel sample of the same frame, all hardware threads:
+    (record, counters, rays per path, oracle image, pixel stride)."""
     from oracle import oracle as O
 
     o = workload.options
     t0 = time.time()
     ref = O.OracleScene.from_view_arrays(workload.scene.view_arrays())
     log(f"[bench] oracle scene built in {time.time() - t0:.1f}s")
-    threads = min(16, O.hardware_threads())
+    hw = O.hardware_threads()
+    threads = hw
     npix = o.width * o.height
     stride = 4096
     for _ in range(3):
         t = time.time()
-        _, ctr = ref.render(o.width, o.height, o.spp, o.max_depth, threads=threads, seed=o.seed, pixel_stride=stride)
+        img, ctr = ref.render(o.width, o.height, o.spp, o.max_depth, threads=threads, seed=o.seed,
+                              pixel_stride=stride)
         dt = time.time() - t
         if dt > target_seconds / 2 or stride == 1:  # keep a sample of at least ~10 s
             break
@@ -80,10 +91,35 @@ def cpu_baseline(workload, target_seconds: float = 20.0):
     paths = int(ctr["paths"])
     rays = int(ctr["extend_rays"]) + int(ctr["shadow_rays"])
     return {
-        "value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-        "sample": f"{'every pixel' if stride == 1 else f'one pixel in {stride}'} of the {o.width}x{o.height} frame, all {o.spp} spp ({paths} paths, "
-                  f"{dt:.1f}s), C++ f64 restatement of the reference CPU path",
-    }, ctr, rays / max(paths, 1)
+        "value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "hardware_threads": hw,
+        "kind": "port",
+        "sample": f"{'every pixel' if stride == 1 else f'one pixel in {stride}'} of the {o.width}x{o.height} frame, "
+                  f"all {o.spp} spp ({paths} paths, {dt:.1f}s), seed {o.seed}, C++ f64 restatement of the reference "
+                  "CPU path",
+    }, ctr, rays / max(paths, 1), img, stride
+
+
+def image_error(gpu_img, ref_img, stride: int, gpu_rays=None, ref_ctr=None) -> dict:
+    """The GPU frame against the oracle's pixels of the same frame and seed (common random
+    numbers: the same per-pixel streams and draw order). `flip_fraction`: pixels some channel
+    of which differs by more than 1e-4 relative -- fp32 rounding moves a pixel mean by ~1e-6,
+    one diverged path by about a sample's value over spp -- an upper bound on the pixels in
+    which a decision (hit, roulette, reflection) went the other way."""
+    import numpy as np
+    g = gpu_img.reshape(-1, 3).astype(np.float64)
+    c = ref_img.reshape(-1, 3)
+    sel = np.arange(len(c)) % stride == 0
+    g, c = g[sel], c[sel]
+    rmse = float(np.sqrt(np.mean((g - c) ** 2)) / np.sqrt(np.mean(c ** 2)))
+    tol = 1e-4 * np.maximum(np.abs(c), 1e-3 * float(np.abs(c).mean())) + 1e-7
+    flips = float((np.abs(g - c) > tol).any(axis=1).mean())
+    out = {"rel_rmse": round(rmse, 7), "flip_fraction": round(flips, 6), "pixels": int(sel.sum())}
+    if gpu_rays is not None and ref_ctr is not None and stride == 1:
+        ext, sh = int(ref_ctr["extend_rays"]), int(ref_ctr["shadow_rays"])
+        out["rays"] = {"gpu_extend": gpu_rays[0], "gpu_shadow": gpu_rays[1], "oracle_extend": ext,
+                       "oracle_shadow": sh,
+                       "rel_diff": round((abs(gpu_rays[0] - ext) + abs(gpu_rays[1] - sh)) / max(ext + sh, 1), 7)}
+    return out
 
 
 def fixture_bytes_per_path():
@@ -171,13 +207,18 @@ def bench_configs(zt, N, W, steps: int = 5):
         for _ in range(steps):
             ctx.render_device(sid, o, tu, d)
             kms.append(ctx.stats()["kernel_ms"])
+        st = ctx.stats()
+        img = ctx.to_host(d, np.zeros((o.height, o.width, 3), np.float32))
         ctx.free(d)
         ctx.close()
         paths = o.width * o.height * o.spp
-        cpu, _, rays = cpu_baseline(wl, target_seconds=6.0)
-        gpu = paths / float(np.median(kms)) / 1e3
-        out[name] = {"paths": paths, "gpu_mpaths_s": round(gpu, 1), "gpu_kernel_ms_median": round(float(np.median(kms)), 3),
-                     "gpu_mrays_s": round(gpu * rays, 1), "cpu": cpu, "gpu_over_cpu": round(gpu / cpu["value"], 1),
+        cpu, ctr, _, ref_img, stride = cpu_baseline(wl, target_seconds=6.0)
+        km = float(np.median(kms))
+        gpu = paths / km / 1e3
+        out[name] = {"paths": paths, "gpu_mpaths_s": round(gpu, 1), "gpu_kernel_ms_median": round(km, 3),
+                     "gpu_mrays_s": round((st["extend_rays"] + st["shadow_rays"]) / km / 1e3, 1),
+                     "cpu": cpu, "gpu_over_cpu": round(gpu / cpu["value"], 1),
+                     "image_error": image_error(img, ref_img, stride, (st["extend_rays"], st["shadow_rays"]), ctr),
                      "tuning": {k: v for k, v in tu.__dict__.items() if v}}
         log(f"[bench] {name}: {out[name]}")
     return out
@@ -281,19 +322,40 @@ def main() -> int:
             kms.append(ctx.stats()["kernel_ms"])
             log(f"[bench] {name} step {i + 1}/{args.steps}: {kms[-1]:.1f} ms")
         wall = time.perf_counter() - t
+        st = ctx.stats()  # the last timed render's device counters
+        rays = st["extend_rays"] + st["shadow_rays"]
         results[name] = {"ms_per_step": wall * 1e3 / args.steps, "kernel_ms": float(np.mean(kms)),
                          "kernel_ms_median": float(np.median(kms)), "mpaths_s": paths * args.steps / wall / 1e6,
                          "mpaths_s_median": paths / float(np.median(kms)) / 1e3,
+                         "mrays_s": rays * args.steps / wall / 1e6,
+                         "extend_rays": st["extend_rays"], "shadow_rays": st["shadow_rays"],
                          "tuning": {k: v for k, v in tu.__dict__.items() if v}}
     img = ctx.to_host(dout, np.zeros((o.height, o.width, 3), np.float32))
     assert np.isfinite(img).all()
     best = max(results, key=lambda k: results[k]["mpaths_s"])
     r = results[best]
 
-    cpu, ctr, rays_per_path = (None, None, None)
+    fp64 = None
+    if not args.no_extra:  # the verification path (the reference's precision) on the same frame
+        d64 = ctx.alloc(o.width * o.height * 3 * 8)
+        tu64 = zt.Tuning(precision=N.PRECISION_FP64)
+        ctx.render_device(sid, o, tu64, d64)
+        k64 = []
+        for _ in range(2):
+            ctx.render_device(sid, o, tu64, d64)
+            k64.append(ctx.stats()["kernel_ms"])
+        ctx.free(d64)
+        fp64 = {"mpaths_s": round(paths / float(np.median(k64)) / 1e3, 2), "kernel_ms_median": round(float(np.median(k64)), 2),
+                "steps": 2, "method": "megakernel, one pixel per lane, two-child BVH"}
+        log(f"[bench] fp64 verification path: {fp64}")
+
+    cpu, ctr, rays_per_path, img_err = None, None, None, None
     if not args.no_cpu:
-        cpu, ctr, rays_per_path = cpu_baseline(w)
+        cpu, ctr, rays_per_path, ref_img, stride = cpu_baseline(w)
         log(f"[bench] cpu baseline {cpu['value']:.3f} Mpaths/s on {cpu['cores']} threads")
+        rm = results.get("megakernel", r)
+        img_err = image_error(img, ref_img, stride, (rm["extend_rays"], rm["shadow_rays"]), ctr)
+        log(f"[bench] image error vs oracle: {img_err}")
     roofline = None
     if "megakernel" in results:
         if ctr is not None:
@@ -323,6 +385,10 @@ def main() -> int:
         t = time.time()
         extra["adjoint"] = bench_adjoint(zt, N, W, steps=args.adjoint_steps)
         log(f"[bench] adjoint leg {time.time() - t:.1f}s: {extra['adjoint']}")
+        if not args.no_cpu:
+            t = time.time()
+            extra["configs"] = bench_configs(zt, N, W)
+            log(f"[bench] configs leg {time.time() - t:.1f}s")
     line = {
         "metric": "Mpaths/s @1080p 64spp (plain vs wavefront-batched); fwd+adjoint iters/s",
         "value": round(r["mpaths_s"], 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
@@ -334,10 +400,14 @@ def main() -> int:
                    "light_sampling": True, "method": best},
         "methods": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
-        "mrays_s": round(r["mpaths_s"] * (rays_per_path or fixture_bytes_per_path()[1]), 2),
+        "mrays_s": round(r["mrays_s"], 2),
+        "mrays_source": "extension + shadow rays counted on the device during the timed renders",
         "scene_upload_ms": round(upload_ms, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "image_error": img_err,
+        "fp64_verification": fp64,
+        "library": N.loaded_library(),
         **extra,
     }
     if dout is not None:

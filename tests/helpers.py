@@ -71,3 +71,45 @@ def to_oracle_rays(r: np.ndarray) -> np.ndarray:
 def block_mean(img: np.ndarray, b: int) -> np.ndarray:
     h, w = img.shape[:2]
     return img[: h // b * b, : w // b * b].reshape(h // b, b, w // b, b, -1).mean(axis=(1, 3))
+
+
+def seed_mean_test(gpu: list, cpu: list, block: int, z: float = 4.0):
+    """Per-block means over independent seeds, GPU vs oracle: returns (beyond, cells, expected).
+
+    `gpu`, `cpu`: renders (H, W, 3) with independent seeds. Each b x b block
+    and channel is one cell; `beyond` counts cells whose mean difference
+    exceeds z standard errors (Welch), `expected` is how many of them the
+    t distribution of that statistic puts beyond z when both estimators have
+    the same mean (the seed-level variances are estimated from few seeds, so
+    that is not the Gaussian 6e-5 per cell)."""
+    from scipy import stats
+    g = np.stack([block_mean(x, block) for x in gpu])
+    c = np.stack([block_mean(x, block) for x in cpu])
+    k1, k2 = len(g), len(c)
+    vg, vc = g.var(axis=0, ddof=1) / k1, c.var(axis=0, ddof=1) / k2
+    se = np.sqrt(vg + vc)
+    diff = np.abs(g.mean(axis=0) - c.mean(axis=0))
+    zz = np.where(se > 0, diff / np.maximum(se, 1e-300), np.where(diff > 1e-6, np.inf, 0))
+    den = vg ** 2 / (k1 - 1) + vc ** 2 / (k2 - 1)
+    df = np.where(den > 0, (vg + vc) ** 2 / np.maximum(den, 1e-300), 1e9)
+    expected = float((2 * stats.t.sf(z, df)).sum())
+    return int((zz > z).sum()), int(zz.size), expected
+
+
+def image_mean_within(gpu: list, cpu: list, z: float = 4.0) -> tuple[bool, np.ndarray]:
+    """Whole-image mean per channel over seeds agrees within z standard errors."""
+    g = np.stack([x.reshape(-1, 3).mean(axis=0) for x in gpu])
+    c = np.stack([x.reshape(-1, 3).mean(axis=0) for x in cpu])
+    se = np.sqrt(g.var(axis=0, ddof=1) / len(g) + c.var(axis=0, ddof=1) / len(c))
+    zz = np.abs(g.mean(axis=0) - c.mean(axis=0)) / np.maximum(se, 1e-300)
+    return bool((zz <= z).all()), zz
+
+
+def strided(img: np.ndarray, stride: int) -> np.ndarray:
+    """The pixels (y*W + x) % stride == 0 of an (H, W, 3) image, as the oracle's pixel_stride renders them."""
+    flat = img.reshape(-1, 3)
+    return flat[np.arange(len(flat)) % stride == 0]
+
+
+def rel_rmse(got: np.ndarray, want: np.ndarray) -> float:
+    return float(np.sqrt(np.mean((got - want) ** 2)) / np.sqrt(np.mean(want ** 2)))

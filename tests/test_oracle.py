@@ -355,7 +355,7 @@ def test_oracle_emitters_exact_and_thread_determinism():  # render.zig:211-228, 
     s.build()
     img, _ = oracle_of(s).render(**opts())
     assert np.array_equal(img, np.broadcast_to([2, 1, 0.5], img.shape))
-    text = open(os.path.join(os.path.dirname(__file__), "golden", "scenes", "glass.txt")).read()
+    text = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "examples", "glass.txt")).read()
     ref = O.OracleScene.parse(text)
     a, _ = ref.render(**opts(threads=1))
     b, _ = ref.render(**opts(threads=4))

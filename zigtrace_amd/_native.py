@@ -12,9 +12,16 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# ZIGTRACE_AMD_LIB selects another build of the same library (e.g. an
-# instrumented or differently configured one); it must export the same ABI.
-LIB_PATH = os.environ.get("ZIGTRACE_AMD_LIB") or os.path.join(HERE, "lib", "libzigtrace_amd.so")
+DEFAULT_LIB_PATH = os.path.join(HERE, "lib", "libzigtrace_amd.so")
+# ZIGTRACE_AMD_LIB selects another build of the same library (an instrumented
+# or differently configured one, exporting the same ABI). It is honoured only
+# together with ZIGTRACE_AMD_ALLOW_OTHER_LIB=1, so a stray variable cannot
+# silently swap the library that tests and benchmarks load.
+_override = os.environ.get("ZIGTRACE_AMD_LIB")
+if _override and os.environ.get("ZIGTRACE_AMD_ALLOW_OTHER_LIB") != "1":
+    raise ImportError(f"ZIGTRACE_AMD_LIB={_override} is set but ZIGTRACE_AMD_ALLOW_OTHER_LIB=1 is not: "
+                      "refusing to load a library other than the in-tree build")
+LIB_PATH = _override or DEFAULT_LIB_PATH
 
 # ---------------------------------------------------------------- status codes (zt_types.h)
 OK = 0
@@ -171,6 +178,16 @@ def lib() -> C.CDLL:
             f.restype = res
             f.argtypes = args
     return _lib
+
+
+def loaded_library() -> dict:
+    """Path and SHA-256 (first 16 hex digits) of the library this process loaded."""
+    import hashlib
+    lib()
+    with open(LIB_PATH, "rb") as f:
+        digest = hashlib.sha256(f.read()).hexdigest()[:16]
+    return {"path": os.path.relpath(LIB_PATH, os.path.dirname(HERE)), "sha256_16": digest,
+            "default": os.path.abspath(LIB_PATH) == os.path.abspath(DEFAULT_LIB_PATH)}
 
 
 def declared_symbols() -> list[str]:

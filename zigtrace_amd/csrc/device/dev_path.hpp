@@ -40,6 +40,36 @@ struct Params {  // per-render constants (render.zig:12-23)
   uint32_t _pad;
 };
 
+// Status words every render kernel receives (`int* status`, 32 bytes,
+// cleared before the launch): word 0 holds error bits, words 2-3 and 4-5 the
+// 64-bit counts of extension rays (each Scene.intersect of render.zig:73) and
+// next-event shadow rays (render.zig:51) the launch traced.
+constexpr size_t kStatusBytes = 32;
+ZT_D void addRayCounts(int* status, uint64_t ext, uint64_t sh) {
+  if (ext) atomicAdd((unsigned long long*)(status + 2), (unsigned long long)ext);
+  if (sh) atomicAdd((unsigned long long*)(status + 4), (unsigned long long)sh);
+}
+
+// Rays traced by one lane, for kernels whose lanes trace independently.
+struct RayTally {
+  uint32_t ext = 0, sh = 0;
+  ZT_D void count(bool shadow) {
+    if (shadow) ++sh;
+    else ++ext;
+  }
+  // Adds the wave's lanes' counts to the status words with one atomic pair
+  // per wave. Every lane of the wave must call it together.
+  ZT_D void flush(int* status) const {
+    uint64_t e = ext, s = sh;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      e += __shfl_xor(e, o);
+      s += __shfl_xor(s, o);
+    }
+    if (__lane_id() == 0) addRayCounts(status, e, s);
+  }
+};
+
 // ---------------------------------------------------------------- camera.zig:59-67 (+ render.zig:138-140)
 // Pixel jitter, lens sample and ray in f64 for both precisions.
 template <typename R>

@@ -10,6 +10,7 @@
 // wave keeps tracing while its lanes' paths have different lengths.
 
 #include <cmath>
+#include <cstring>
 
 #include "../device/dev_path.hpp"
 #include "../device/dev_wave.hpp"
@@ -39,7 +40,7 @@ ZT_D uint32_t bandStart(uint32_t band, uint32_t ntiles) {
 // One pixel: all its samples from its own stream, in order (render.zig:133-144).
 template <typename R>
 ZT_D void renderPixel(const SceneView<R>& sc, const Params& pr, const LaneStack& stk, uint32_t x, uint32_t y,
-                      R* __restrict__ out, int* overflow) {
+                      R* __restrict__ out, int* overflow, RayTally& rays) {
   const uint64_t pix = (uint64_t)y * pr.width + x;
   Rng rng = rngForStream(pr.seed, pix);
   V3<R> sum = splat(R(0));
@@ -55,6 +56,7 @@ ZT_D void renderPixel(const SceneView<R>& sc, const Params& pr, const LaneStack&
       const bool sh = phase == kShadow;
       const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
                                       sh ? st.sh_excl : -1, sh, overflow);
+      rays.count(sh);
       phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
     }
     if (phase == kFinish) {
@@ -82,6 +84,7 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
   const uint32_t tid = threadIdx.x;
   const LaneStack stk = laneStack(lds, cap, spill);
   int overflow = 0;
+  RayTally rays;
   const uint32_t home = blockIdx.x % 8;
   for (uint32_t k = 0; k < 8; ++k) {
     const uint32_t band = (home + k) % 8;
@@ -94,9 +97,10 @@ ZT_D void megakernelBody(const SceneView<R>& sc, const Params& pr, R* __restrict
       if (tile >= hi) break;
       const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
       const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
-      if (x < pr.width && y < pr.height) renderPixel<R>(sc, pr, stk, x, y, out, &overflow);
+      if (x < pr.width && y < pr.height) renderPixel<R>(sc, pr, stk, x, y, out, &overflow, rays);
     }
   }
+  rays.flush(err);
   if (overflow) atomicOr(err, 1);
 }
 
@@ -184,6 +188,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   Path<float> st;
   Trav4 tr;
   int overflow = 0;
+  uint64_t n_ext = 0, n_sh = 0;  // rays started by the wave (wave-uniform)
   for (;;) {
     // Pixels for lanes without one, from the wave's tile (then the next tile).
     for (;;) {
@@ -229,7 +234,8 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       }
     }
     // Lanes with no ray in flight advance their path to its next ray.
-    if (has && !tracing) {
+    const bool idle = has && !tracing;
+    if (idle) {
       const SceneView<float>& sc = src.view(si);
       const Params& pr = src.params(si);
       for (;;) {
@@ -265,6 +271,11 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
         break;
       }
     }
+    if constexpr (!kCost) {
+      const uint64_t started = __ballot(idle && tracing), shadow = __ballot(idle && tracing && sh);
+      n_ext += (uint64_t)__popcll(started & ~shadow);
+      n_sh += (uint64_t)__popcll(shadow);
+    }
     if (__ballot(tracing) == 0) {
       // Every lane is without a pixel (a lane with one always has a ray now).
       if (tiles_done) break;
@@ -289,6 +300,8 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
       }
     }
   }
+  if constexpr (!kCost)
+    if (lane == 0) addRayCounts(err, n_ext, n_sh);
   if (overflow) atomicOr(err, 1);
 }
 
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(256) void orderTiles(const uint32_t* cost, uint32_t
 // traceWith for caller-given rays and RNG states (render.zig:61-102).
 template <typename R>
 ZT_D void traceOne(const SceneView<R>& sc, const Params& pr, const LaneStack& stk, const zt_ray* rays, uint64_t i,
-                   uint64_t* states, double* out, int* overflow) {
+                   uint64_t* states, double* out, int* overflow, RayTally& tally) {
   Rng rng{states[4 * i], states[4 * i + 1], states[4 * i + 2], states[4 * i + 3]};
   Path<R> st;
   st.ro = mk((R)rays[i].origin[0], (R)rays[i].origin[1], (R)rays[i].origin[2]);
@@ -367,6 +380,7 @@ ZT_D void traceOne(const SceneView<R>& sc, const Params& pr, const LaneStack& st
     const bool sh = phase == kShadow;
     const HitRec<R> h = rayQuery<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, sh ? st.sh_tmax : R(INFINITY), st.self,
                                  sh ? st.sh_excl : -1, sh, overflow);
+    tally.count(sh);
     phase = sh ? afterShadow<R>(h, st) : shade<R>(sc, pr, h, rng, st);
   }
   out[3 * i + 0] = st.radiance.x;
@@ -384,8 +398,10 @@ __global__ __launch_bounds__(256) void traceKernel(SceneView<R> sc, Params pr, c
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, kLds, spill);
   int overflow = 0;
+  RayTally tally;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    traceOne<R>(sc, pr, stk, rays, i, states, out, &overflow);
+    traceOne<R>(sc, pr, stk, rays, i, states, out, &overflow, tally);
+  tally.flush(err);
   if (overflow) atomicOr(err, 1);
 }
 
@@ -668,13 +684,18 @@ template void launchMegakernel<float>(zt_ctx*, const DevScene&, const zt_render_
 template void launchMegakernel<double>(zt_ctx*, const DevScene&, const zt_render_options&, const zt_gpu_tuning&,
                                        double*);
 
-void resetErr(zt_ctx* ctx) { ZT_HIP(hipMemsetAsync(ctx->errflag.p, 0, sizeof(int) * 4, ctx->stream)); }
+void resetErr(zt_ctx* ctx) { ZT_HIP(hipMemsetAsync(ctx->errflag.p, 0, kStatusBytes, ctx->stream)); }
 
 void checkErr(zt_ctx* ctx) {
-  int e = 0;
-  ZT_HIP(hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  int w[kStatusBytes / sizeof(int)] = {};
+  ZT_HIP(hipMemcpyAsync(w, ctx->errflag.p, kStatusBytes, hipMemcpyDeviceToHost, ctx->stream));
   ZT_HIP(hipStreamSynchronize(ctx->stream));
-  if (e) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+  uint64_t ext = 0, sh = 0;
+  std::memcpy(&ext, w + 2, 8);
+  std::memcpy(&sh, w + 4, 8);
+  ctx->stats.extend_rays = ext;
+  ctx->stats.shadow_rays = sh;
+  if (w[0]) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
 }
 
 }  // namespace ztg

@@ -16,7 +16,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "../device/dev_trace.hpp"
+#include "../device/dev_path.hpp"
 #include "ctx.hpp"
 
 using namespace ztd;
@@ -455,7 +455,7 @@ int zt_gpu_create(int device, zt_ctx** out) {
     ZT_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ZT_HIP(hipEventCreate(&ctx->ev0));
     ZT_HIP(hipEventCreate(&ctx->ev1));
-    ctx->errflag.alloc(sizeof(int) * 4);
+    ctx->errflag.alloc(ztd::kStatusBytes);
   });
   if (rc != ZT_OK) {
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

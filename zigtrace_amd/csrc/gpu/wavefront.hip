@@ -41,7 +41,8 @@ struct WfShared {
   double* hits_d;     // per slot: t in f64 for the fp64 path
   uint32_t* qext[2];  // extension queues (double-buffered)
   uint32_t* qsh;      // shadow queue
-  // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64)
+  // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64),
+  // [8..9] extension rays traced (u64), [10..11] shadow rays traced (u64)
   uint32_t* counters;
   int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
@@ -150,6 +151,7 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
     w.counters[2] = 0;        // shadow queue
   }
   const uint32_t n = shadow ? w.counters[2] : w.counters[cur];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *(unsigned long long*)&w.counters[shadow ? 10 : 8] += n;
   const uint32_t* q = shadow ? w.qsh : w.qext[cur];
   const uint32_t stride = gridDim.x * blockDim.x;
   int overflow = 0;
@@ -222,6 +224,14 @@ __global__ __launch_bounds__(256) void wfResolve(WfShared<R> w, int nxt) {
       w.slots[s] = sl;
     }
     append(w.qext[nxt], &w.counters[nxt], s, to_ext);
+  }
+}
+
+// Adds the loop's ray counts and error flag to the launch status words.
+__global__ void wfReport(const uint32_t* counters, int* status) {
+  if (threadIdx.x == 0) {
+    addRayCounts(status, *(const unsigned long long*)&counters[8], *(const unsigned long long*)&counters[10]);
+    if (counters[4]) atomicOr(status, 1);
   }
 }
 
@@ -338,11 +348,9 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
-  ctx->stats.launches += batches * kBatch * 4;
-  uint32_t flag = 0;
-  ZT_HIP(hipMemcpyAsync(&flag, w.counters + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  ZT_HIP(hipStreamSynchronize(ctx->stream));
-  if (flag) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+  ctx->stats.launches += batches * kBatch * 4 + 1;
+  hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, ctx->stream, w.counters, ctx->errflag.as<int>());
+  ZT_HIP(hipGetLastError());
 }
 template void runWavefront<float>(zt_ctx*, const std::vector<WfScene<float>>&, const zt_gpu_tuning&);
 template void runWavefront<double>(zt_ctx*, const std::vector<WfScene<double>>&, const zt_gpu_tuning&);
@@ -396,6 +404,7 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
       return out_on_device ? outs[i] : (void*)((char*)ctx->out.p + offs[i]);
     };
     ctx->stats.launches = 0;
+    resetErr(ctx);
     ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     uint64_t first = 0, paths = 0;
     auto build = [&](auto tag) {
@@ -422,9 +431,7 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
         if (wave) {
           std::vector<WaveJob> jobs(n);
           for (uint32_t i = 0; i < n; ++i) jobs[i] = WaveJob{scenes[i].view, scenes[i].pr, scenes[i].out, 0, 0};
-          resetErr(ctx);
           launchWaveJobs(ctx, jobs, tu);
-          checkErr(ctx);
           return;
         }
       }
@@ -436,7 +443,7 @@ extern "C" int zt_gpu_render_batch(zt_ctx* ctx, uint32_t n, const uint32_t* scen
     if (!out_on_device)
       for (uint32_t i = 0; i < n; ++i)
         ZT_HIP(hipMemcpyAsync(outs[i], devOut(i), bytes[i], hipMemcpyDeviceToHost, ctx->stream));
-    ZT_HIP(hipStreamSynchronize(ctx->stream));
+    checkErr(ctx);
     float ms = 0;
     ZT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms = ms;

@@ -23,7 +23,7 @@ from .render import Options
 from .scene import CameraConfig, Scene, dielectric, emissive, lambertian, metal, sky, solid
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SCENE_DIR = os.path.join(os.path.dirname(HERE), "tests", "golden", "scenes")
+SCENE_DIR = os.path.join(os.path.dirname(HERE), "examples")
 
 
 @dataclass
