@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
         _lib.orc_scene_flat.argtypes = [C.c_void_p] * 6
         _lib.orc_scene_bvh.argtypes = [C.c_void_p] * 3
         _lib.orc_scene_lights.argtypes = [C.c_void_p] * 6
+        _lib.orc_set_perturbation.argtypes = [C.c_double]
     return _lib
 
 
@@ -419,6 +420,24 @@ def render_grad(scene: "OracleScene", width, height, spp, max_depth, offsets, di
     if rc:
         raise ValueError("InvalidOptions" if rc == 1 else "too many parameters")
     return img, grad[:nparams]
+
+
+class rounding_perturbation:
+    """Context manager: renders inside it perturb every hit point and scattered
+    direction by a relative (-rel, rel) (deterministic per path vertex, no RNG
+    draw consumed) -- the image noise that rounding at that scale alone causes
+    with common random numbers."""
+
+    def __init__(self, rel: float):
+        self.rel = rel
+
+    def __enter__(self):
+        lib().orc_set_perturbation(self.rel)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_perturbation(0.0)
+        return False
 
 
 def hardware_threads() -> int:

@@ -192,6 +192,7 @@ struct Hit {
   V3 p, n;
   bool front;
   uint32_t material;
+  double curv = 0;  // 1/radius of a sphere hit, 0 for flat shapes (used only by the rounding perturbation)
 };
 static inline Hit makeHit(const Ray& r, double t, V3 outward, uint32_t material) {  // geometry.zig:20-29
   const bool front = dot(r.d, outward) < 0;
@@ -218,6 +219,7 @@ static inline bool sphereIntersect(const Shape& s, const Ray& r, double tmin, do
   }
   const V3 outward = vdiv(sub(at(r, t), s.a), radius);
   *h = makeHit(r, t, outward, s.material);
+  h->curv = 1.0 / radius;
   return true;
 }
 static inline bool planeIntersect(const Shape& s, const Ray& r, double tmin, double tmax, Hit* h) {
@@ -805,6 +807,27 @@ static V3 directLight(const Scene& scene, V3 albedo, const Hit& hit, Rng* rng, C
   const double sc = cos_s / (M_PI * ls.pdf) * powerWeight(ls.pdf, bsdf_pdf);
   return scale(mul(albedo, ls.radiance), sc);
 }
+// Rounding-scale perturbation (off unless a test sets it): every scattered
+// direction is multiplied component-wise by 1 + e, and every hit point moved
+// by that much within its tangent plane, e uniform in (-rel, rel) from a hash
+// of the path's RNG state and depth (no draw is consumed, so the path keeps
+// its stream); on a sphere the normal follows the surface to the moved point.
+// The component along the normal is left out: an offset off the surface is
+// what t_min (and, on the device, skipping the primitive a ray leaves)
+// absorbs, not a change of the path. With rel at the scale of
+// fp32 rounding (2^-24 .. 2^-22) it measures how far rounding alone moves a
+// converged image with common random numbers: the noise floor any fp32
+// renderer of the same estimator has on that scene.
+static double g_perturb_rel = 0;
+static inline double perturbed(uint64_t key, double v) {
+  uint64_t z = key;
+  const double u = (double)(splitMix64(&z) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  return v * (1.0 + u * g_perturb_rel);
+}
+static inline V3 perturbed3(uint64_t key, V3 v) {
+  return v3(perturbed(key, v.x), perturbed(key + 1, v.y), perturbed(key + 2, v.z));
+}
+
 static V3 traceWith(const Scene& scene, Ray r, Rng* rng, uint64_t max_depth, bool light_sampling,
                     Counters* ctr) {  // render.zig:61-102
   const bool use_nee = light_sampling && !scene.lights.items.empty();
@@ -816,6 +839,13 @@ static V3 traceWith(const Scene& scene, Ray r, Rng* rng, uint64_t max_depth, boo
     if (ctr) ++ctr->extend_rays;
     if (!scene.intersect(r, kRayEpsilon, kInf, &hit, ctr))
       return add(radiance, mul(throughput, scene.background.radiance(r.d)));
+    if (g_perturb_rel > 0) {  // within the tangent plane: off-surface error is what t_min absorbs
+      V3 d = sub(perturbed3(rng->s[0] ^ (depth * 0x9E3779B97F4A7C15ull), hit.p), hit.p);
+      d = sub(d, scale(hit.n, dot(d, hit.n)));
+      hit.p = add(hit.p, d);
+      // On a sphere the normal follows the surface to the moved point.
+      if (hit.curv != 0) hit.n = normalize(add(hit.n, scale(d, hit.front ? hit.curv : -hit.curv)));
+    }
     const Material& mat = scene.materials[hit.material];
     const V3 le = emitted(mat, hit);
     if (maxComponent(le) > 0) {
@@ -827,6 +857,7 @@ static V3 traceWith(const Scene& scene, Ray r, Rng* rng, uint64_t max_depth, boo
     V3 att;
     Ray next;
     if (!scatter(mat, r, hit, rng, &att, &next)) break;
+    if (g_perturb_rel > 0) next.d = perturbed3(rng->s[1] ^ (depth * 0xD1B54A32D192ED03ull), next.d);
     throughput = mul(throughput, att);
     prev_pdf = (use_nee && mat.kind == kLambertian) ? std::fmax(dot(next.d, hit.n), 0.0) / M_PI : 0;
     prev_origin = hit.p;
@@ -1344,7 +1375,11 @@ int parseScene(const char* src, size_t len, Scene* scene, uint64_t* line_out, st
 
 extern "C" {
 
+// Test-only: the rounding-scale perturbation of traceWith (0 = off, the default).
+void orc_set_perturbation(double rel) { orc::g_perturb_rel = rel; }
+
 // ---- RNG (rng.zig)
+
 void orc_splitmix64(uint64_t state, uint64_t n, uint64_t* out) {
   for (uint64_t i = 0; i < n; ++i) out[i] = splitMix64(&state);
 }

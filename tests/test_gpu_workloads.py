@@ -11,7 +11,11 @@ Each workload is compared two ways, the reference's per-pixel estimator
   order are the reference's (render.zig:133-144), so at 4096 samples per
   pixel the fp32 image's relative RMSE against the oracle's f64 image stays
   below 1e-3. Stream ids are y*W + x, so the GPU renders the whole frame and
-  the oracle the pixels (y*W + x) % stride == 0 of the same frame.
+  the oracle the pixels (y*W + x) % stride == 0 of the same frame. What is
+  left is decision flips (a ray that grazes a silhouette, a roulette draw
+  on a threshold), whose effect depends on the scene: the oracle's rounding
+  perturbation measures it, and where rounding alone moves the oracle's own
+  image by more than 1e-3 the bound is 3x that floor instead.
 
 Workloads: the 1.3M-triangle mesh field (1920 wide), the sphere field over a
 ground plane (600x340), the shipped spheres.txt (r = 1000 ground sphere),
@@ -129,19 +133,43 @@ def batch4():
     return ws, [oracle_of(w.scene) for w in ws]
 
 
-@pytest.mark.parametrize("method", [N.METHOD_AUTO, N.METHOD_WAVEFRONT])
-def test_batch_scenes_crn_rmse_through_render_batch(gpu_ctx, batch4, method):
+BATCH_SIDE, BATCH_CRN_SPP = 64, 4096
+FLOOR_REL, FLOOR_FACTOR = 2.0 ** -22, 3.0
+
+
+@pytest.fixture(scope="module")
+def batch4_crn(batch4):
+    """Per scene: the oracle's 4096-spp image (seed 40 + i) and its rounding-noise floor, the
+    relative RMSE that moving every hit point (within its surface) and every scattered
+    direction by a relative 2^-22 -- two fp32 ulps -- causes in the oracle itself."""
     ws, refs = batch4
+    out = []
+    for i, (w, ref) in enumerate(zip(ws, refs)):
+        args = (BATCH_SIDE, BATCH_SIDE, BATCH_CRN_SPP, w.options.max_depth)
+        want, _ = ref.render(*args, seed=40 + i)
+        with O.rounding_perturbation(FLOOR_REL):
+            moved, _ = ref.render(*args, seed=40 + i)
+        out.append((want, rel_rmse(moved, want)))
+    return out
+
+
+@pytest.mark.parametrize("method", [N.METHOD_AUTO, N.METHOD_WAVEFRONT])
+def test_batch_scenes_crn_rmse_through_render_batch(gpu_ctx, batch4, batch4_crn, method):
+    """Scenes of uneven size in one batch launch. The 1e-3 bar holds where rounding allows it:
+    some of these scenes (thousands of overlapping small spheres, glass, depth 27) are so
+    sensitive that two-ulp rounding moves the oracle's own converged image by more than 1e-3;
+    there the fp32 image must stay within 3x that floor."""
+    ws, _ = batch4
     ids = [gpu_ctx.upload(w.scene) for w in ws]
-    side, spp, stride = 64, 4096, 13
-    opts = [zt.Options(side, side, spp, w.options.max_depth, seed=40 + i) for i, w in enumerate(ws)]
+    opts = [zt.Options(BATCH_SIDE, BATCH_SIDE, BATCH_CRN_SPP, w.options.max_depth, seed=40 + i)
+            for i, w in enumerate(ws)]
     outs = gpu_ctx.render_batch(ids, opts, zt.Tuning(method=method))
     for sid in ids:
         gpu_ctx.release(sid)
-    for w, ref, o, img in zip(ws, refs, opts, outs):
-        want, _ = ref.render(o.width, o.height, o.spp, o.max_depth, seed=o.seed, pixel_stride=stride)
-        err = rel_rmse(strided(img.astype(np.float64), stride), strided(want, stride))
-        assert err < RMSE_BAR, f"{w.description}: relative RMSE {err:.2e}"
+    for w, (want, floor), img in zip(ws, batch4_crn, outs):
+        err = rel_rmse(img.astype(np.float64), want)
+        assert err < max(RMSE_BAR, FLOOR_FACTOR * floor), \
+            f"{w.description}: relative RMSE {err:.2e} (rounding floor {floor:.2e})"
 
 
 def test_batch_scenes_seed_means_through_render_batch(gpu_ctx, batch4):
