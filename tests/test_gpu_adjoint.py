@@ -63,6 +63,52 @@ def test_fp32_adjoint_close_to_oracle(gpu_ctx):
     assert np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g) < 2e-3
 
 
+@pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
+def test_backward_bits_identical_across_calls_and_tile_shapes(gpu_ctx, precision):
+    """No atomics: per-lane columns, fixed 8x8 pixel groups and a fixed-order reduction.
+    The 16x16 fp64 tile keeps its columns in global memory instead of LDS: same bits."""
+    w = W.diff_box(width=72, height=56, spp=16, max_depth=8)  # groups straddle the tile grid's edge
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options, precision=precision)
+    p = box_params(dr)
+    rng = np.random.default_rng(12)
+    dimg = rng.uniform(-1, 1, (56, 72, 3)).astype(np.float32)
+    base = dr.backward(p, dimg)
+    assert np.any(base != 0)
+    for tile in (None, (8, 8), (16, 8), (8, 16), (16, 16), (32, 8)):
+        g = dr.backward(p, dimg, tile=tile)
+        assert np.array_equal(g.view(np.uint64), base.view(np.uint64)), tile
+    dr.release()
+
+
+def test_fp64_backward_with_global_columns_equals_oracle(gpu_ctx):
+    """25 parameter materials (75 parameters): the fp64 columns do not fit in LDS and live
+    in global memory; the gradient still equals the oracle's dual numbers."""
+    rng = np.random.default_rng(3)
+    s = zt.Scene()
+    mats = [s.add_material(zt.lambertian(tuple(rng.uniform(0.2, 0.9, 3)))) for _ in range(24)]
+    lamp = s.add_material(zt.emissive((6.0, 5.0, 4.0)))
+    for k in range(24):
+        s.add_sphere((rng.uniform(-3, 3), rng.uniform(0.3, 2), rng.uniform(-3, 0)), 0.35, mats[k])
+    s.add_plane((0, 0, 0), (0, 1, 0), mats[0])
+    s.add_sphere((0, 4, -1.5), 0.6, lamp)
+    s.camera = zt.CameraConfig((0, 1.5, 6), (0, 1, -1.5), (0, 1, 0), 45.0)
+    s.background = zt.solid((0.1, 0.1, 0.12))
+    s.build()
+    o = zt.Options(40, 32, 8, 6)
+    dr = DiffRenderer(gpu_ctx, s, o, precision=N.PRECISION_FP64)
+    assert dr.num_params == 75
+    p = dr.get_params()
+    dimg = rng.uniform(-1, 1, (32, 40, 3)).astype(np.float32)
+    g = dr.backward(p, dimg)
+    a = s.view_arrays()
+    _, ref_g = O.render_grad(O.OracleScene.from_view_arrays(a), 40, 32, 8, 6, dr.offsets, dimg.astype(np.float64))
+    dr.release()
+    # fp64 on both sides; a path whose sine or cosine differs in the last ulp from the
+    # oracle's may end elsewhere, and the random-sign dL/dimage makes some components sums
+    # of cancelling terms: 1e-6 relative (fp32 agrees to 2e-3 at best).
+    assert np.allclose(g, ref_g, rtol=1e-6, atol=1e-7 * np.abs(ref_g).max())
+
+
 def fd_grad(dr, p, o, dimg, h):
     g = np.zeros(p.size)
     for i in range(p.size):

@@ -56,13 +56,19 @@ class DiffRenderer:
                                           N.ptr(p), N.ptr(out), 0))
         return out
 
-    def backward(self, params: np.ndarray, dl_dimage: np.ndarray, options: Options | None = None) -> np.ndarray:
+    def backward(self, params: np.ndarray, dl_dimage: np.ndarray, options: Options | None = None,
+                 tile: tuple[int, int] | None = None) -> np.ndarray:
+        """dLoss/dparams; `tile` (multiples of 8) sets the backward's workgroup tile, which
+        does not change the result's bits."""
         o = options or self.options
         p = np.ascontiguousarray(params, dtype=np.float32)
         g = np.ascontiguousarray(dl_dimage, dtype=np.float32)
         assert g.size == o.width * o.height * 3
         out = np.zeros(max(self.num_params, 1), dtype=np.float64)
-        self._check(_lib().zt_gpu_backward(self.ctx.handle, self.sid, N.ptr(o.record()), N.ptr(self.tuning.record()),
+        t = self.tuning
+        if tile is not None:
+            t = Tuning(**{**t.__dict__, "tile_w": tile[0], "tile_h": tile[1]})
+        self._check(_lib().zt_gpu_backward(self.ctx.handle, self.sid, N.ptr(o.record()), N.ptr(t.record()),
                                            N.ptr(p), N.ptr(g), 0, N.ptr(out)))
         return out[: self.num_params]
 

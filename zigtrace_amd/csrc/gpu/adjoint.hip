@@ -20,8 +20,14 @@
 //
 // Each lane keeps a short list of the albedo materials its current path has
 // visited (with counts) and of the emitters it has drawn from; at the end of
-// a finite sample the list is divided through and added to a per-workgroup
-// LDS accumulator, which is flushed to global memory once per workgroup.
+// a finite sample the list is divided through and added to the lane's own
+// column of partial gradients (one value per parameter), with plain loads and
+// stores: no atomics, so the bits do not depend on scheduling. Pixels are
+// reduced in fixed 8x8 groups of the image: once a group's 64 lanes are done,
+// lane l sums parameter l (l + 64, ...) over the group's columns in lane
+// order, in double, into the group's row of partials; a second kernel sums
+// the groups' rows in a fixed order. The gradient is therefore the same bits
+// for every call and every launch shape (tile shape, grid size).
 
 #include <algorithm>
 #include <cmath>
@@ -37,6 +43,13 @@ namespace ztg {
 
 static constexpr double kRayEpsA = 1e-4;  // render.zig:26
 static constexpr int kEmitterSlots = 4;
+// Row stride of the per-lane columns (parameter p of lane l at p * kColStride
+// + l): 65 words, so the reduction's lanes (one parameter each, walking the
+// 64 lanes of a row) hit 64 different LDS banks.
+static constexpr int kColStride = 65;
+// Parameters whose columns fit in LDS (kColStride * sizeof(R) bytes each per
+// wave): beyond this the columns live in a global buffer.
+static constexpr size_t kLdsColBytes = 32 * 1024;
 
 template <typename R, int K>
 struct AdjointObserver {
@@ -51,25 +64,30 @@ struct AdjointObserver {
   V3<R> nee_x, nee_dle;
   int nee_mat;
   int overflow;
-  R* lds_grad;
+  R* col;  // this lane's column: parameter p at col[p * kColStride]
   const int* offsets;
 
   ZT_D void reset() {
     n = 0;
     ne = 0;
   }
+  ZT_D void add(int off, V3<R> v) {
+    col[(off + 0) * kColStride] += v.x;
+    col[(off + 1) * kColStride] += v.y;
+    col[(off + 2) * kColStride] += v.z;
+  }
   ZT_D void event(V3<R> x) {
     const V3<R> xd = mul(delta, x);
 #pragma unroll
     for (int i = 0; i < K; ++i)
-      if (i < n) g[i] = add(g[i], scale(xd, R(cnt[i])));
+      if (i < n) g[i] = ztd::add(g[i], scale(xd, R(cnt[i])));
   }
   ZT_D void addEmitter(int e, V3<R> v) {
     bool done = false;
 #pragma unroll
     for (int i = 0; i < kEmitterSlots; ++i) {
       if (!done && i < ne && emat[i] == e) {
-        eg[i] = add(eg[i], v);
+        eg[i] = ztd::add(eg[i], v);
         done = true;
       }
     }
@@ -89,10 +107,7 @@ struct AdjointObserver {
   }
   ZT_D void flushEmitter(int e, V3<R> v) {
     const int off = offsets[e];
-    if (off < 0) return;
-    atomicAdd(&lds_grad[off + 0], v.x);
-    atomicAdd(&lds_grad[off + 1], v.y);
-    atomicAdd(&lds_grad[off + 2], v.z);
+    if (off >= 0) add(off, v);
   }
   // NoObserver hooks (see dev_path.hpp).
   ZT_D void miss(V3<R> x) { event(x); }
@@ -142,9 +157,9 @@ struct AdjointObserver {
           const V3<R> a = sc.mats[mat[i]].color;
           // Zero-albedo channels carry no contribution to divide (0/0): their
           // gradient is left at 0.
-          if (a.x != 0) atomicAdd(&lds_grad[off + 0], g[i].x / a.x);
-          if (a.y != 0) atomicAdd(&lds_grad[off + 1], g[i].y / a.y);
-          if (a.z != 0) atomicAdd(&lds_grad[off + 2], g[i].z / a.z);
+          if (a.x != 0) col[(off + 0) * kColStride] += g[i].x / a.x;
+          if (a.y != 0) col[(off + 1) * kColStride] += g[i].y / a.y;
+          if (a.z != 0) col[(off + 2) * kColStride] += g[i].z / a.z;
         }
       }
     }
@@ -154,54 +169,84 @@ struct AdjointObserver {
   }
 };
 
-template <typename R, int K>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 8))) void adjointKernel(SceneView<R> sc, Params pr, const float* __restrict__ dimg,
-                                                     const int* __restrict__ offsets, int nparams,
-                                                     R* __restrict__ grad, uint32_t tile_w, uint32_t tiles_x,
-                                                     uint32_t ntiles, int* spill, int* err) {
+// Path replay of the pixels of 8x8 groups. A workgroup of tile_w x tile_h
+// pixels (multiples of 8, one wave per group) works through the tiles of the
+// image in turn (persistent grid: tile blockIdx.x, then + gridDim.x, ...);
+// every group's row of partials depends only on its pixels. `gcols`: the
+// per-lane columns in global memory (one block of kColStride * nparams words
+// per wave of the grid) when they do not fit in LDS (kLdsCols false).
+template <typename R, int K, bool kLdsCols>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) void adjointKernel(
+    SceneView<R> sc, Params pr, const float* __restrict__ dimg, const int* __restrict__ offsets, int nparams,
+    R* __restrict__ gcols, double* __restrict__ partial, uint32_t tile_w, uint32_t tile_h, uint32_t tiles_x,
+    uint32_t ntiles, uint32_t groups_x, int cap, int* spill, int* err) {
   extern __shared__ int lds[];
-  const uint32_t tid = threadIdx.x;
-  const LaneStack stk = laneStack<64>(lds, kLds, spill);  // workgroups of one wave (launchAdjoint)
-  R* lg = (R*)(lds + 2 * kLds * blockDim.x);
-  for (int i = (int)tid; i < nparams; i += (int)blockDim.x) lg[i] = R(0);
-  __syncthreads();
-  const uint32_t tile = blockIdx.x;
-  const uint32_t tile_h = blockDim.x / tile_w;
-  const uint32_t x = (tile % tiles_x) * tile_w + tid % tile_w;
-  const uint32_t y = (tile / tiles_x) * tile_h + tid / tile_w;
-  int overflow = 0;
-  if (x < pr.width && y < pr.height) {
-    const uint64_t pix = (uint64_t)y * pr.width + x;
-    Rng rng = rngForStream(pr.seed, pix);
-    AdjointObserver<R, K> obs;
-    const R inv_spp = R(1) / R(pr.spp);
-    obs.delta = mk(R(dimg[3 * pix + 0]) * inv_spp, R(dimg[3 * pix + 1]) * inv_spp, R(dimg[3 * pix + 2]) * inv_spp);
-    obs.lds_grad = lg;
-    obs.offsets = offsets;
-    obs.overflow = 0;
-    Path<R> st;
-    for (uint32_t sample = 0; sample < pr.spp; ++sample) {
-      obs.reset();
-      int phase = beginPath<R>(sc, pr, x, y, rng, st);
-      while (phase != kFinish) {
-        const bool sh = phase == kShadow;
-        const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, R(kRayEpsA), sh ? st.sh_tmax : R(INFINITY),
-                                     st.self, sh ? st.sh_excl : -1, sh, &overflow);
-        phase = sh ? afterShadow<R>(h, st, &obs) : shade<R>(sc, pr, h, rng, st, &obs);
-      }
-      if (isFinite(st.radiance)) obs.flush(sc);  // the forward drops non-finite samples (render.zig:141-142)
-    }
-    overflow |= obs.overflow << 1;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t waves = blockDim.x >> 6;
+  const LaneStack stk = laneStack(lds, cap, spill);
+  R* cols;  // this wave's columns
+  if constexpr (kLdsCols) {
+    cols = (R*)(lds + stackBytes(cap, (int)blockDim.x) / sizeof(int)) + (size_t)wave * kColStride * nparams;
+  } else {
+    cols = gcols + ((size_t)blockIdx.x * waves + wave) * kColStride * nparams;
   }
-  __syncthreads();
-  for (int i = (int)tid; i < nparams; i += (int)blockDim.x)
-    if (lg[i] != R(0)) atomicAdd(&grad[i], lg[i]);
+  const uint32_t gw = tile_w / 8;  // groups per tile row
+  int overflow = 0;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // This wave's group within the tile, and its lane's pixel.
+    const uint32_t gx = (tile % tiles_x) * gw + wave % gw;
+    const uint32_t gy = (tile / tiles_x) * (tile_h / 8) + wave / gw;
+    const uint32_t x = gx * 8 + lane % 8, y = gy * 8 + lane / 8;
+    for (int p = 0; p < nparams; ++p) cols[p * kColStride + lane] = R(0);
+    if (x < pr.width && y < pr.height) {
+      const uint64_t pix = (uint64_t)y * pr.width + x;
+      Rng rng = rngForStream(pr.seed, pix);
+      AdjointObserver<R, K> obs;
+      const R inv_spp = R(1) / R(pr.spp);
+      obs.delta = mk(R(dimg[3 * pix + 0]) * inv_spp, R(dimg[3 * pix + 1]) * inv_spp, R(dimg[3 * pix + 2]) * inv_spp);
+      obs.col = cols + lane;
+      obs.offsets = offsets;
+      obs.overflow = 0;
+      Path<R> st;
+      for (uint32_t sample = 0; sample < pr.spp; ++sample) {
+        obs.reset();
+        int phase = beginPath<R>(sc, pr, x, y, rng, st);
+        while (phase != kFinish) {
+          const bool sh = phase == kShadow;
+          const HitRec<R> h = query<R>(sc, stk, st.ro, sh ? st.sh_d : st.rd, R(kRayEpsA),
+                                       sh ? st.sh_tmax : R(INFINITY), st.self, sh ? st.sh_excl : -1, sh, &overflow);
+          phase = sh ? afterShadow<R>(h, st, &obs) : shade<R>(sc, pr, h, rng, st, &obs);
+        }
+        if (isFinite(st.radiance)) obs.flush(sc);  // the forward drops non-finite samples (render.zig:141-142)
+      }
+      overflow |= obs.overflow << 1;
+    }
+    __syncthreads();  // every lane's column of this group is written
+    if (gx < groups_x && gy * 8 < pr.height) {
+      double* row = partial + ((size_t)gy * groups_x + gx) * nparams;
+      for (int p = (int)lane; p < nparams; p += 64) {
+        double s = 0;
+        for (int j = 0; j < 64; ++j) s += (double)cols[p * kColStride + j];
+        row[p] = s;
+      }
+    }
+    __syncthreads();  // the columns are reused by the next tile
+  }
   if (overflow) atomicOr(err, overflow);
 }
 
-template <typename R>
-__global__ void zeroFill(R* p, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = R(0);
+// dL/dparams[p] = the sum over groups of partial[g][p]: one wave per
+// parameter, lane l summing groups l, l + 64, ... in order, then a fixed
+// butterfly across the lanes.
+__global__ __launch_bounds__(64) void reduceGroups(const double* __restrict__ partial, int nparams, uint32_t ngroups,
+                                                   double* __restrict__ out) {
+  const int p = (int)blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  double s = 0;
+  for (uint32_t g = lane; g < ngroups; g += 64) s += partial[(size_t)g * nparams + p];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[p] = s;
 }
 
 // Rebuilds the light table from the emissive candidates with the current
@@ -252,22 +297,86 @@ void setParams(zt_ctx* ctx, DevScene& s, const float* params) {
   ZT_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+// Traversal stack entries a query of this scene can hold at once: a
+// two-child walk pushes at most depth + 1, a four-wide one 3 * depth + 1.
+int stackNeed(const DevScene& s, bool four_wide) {
+  if (s.brute_force || s.num_nodes == 0) return 0;
+  return four_wide ? 3 * s.tree_depth4 + 1 : s.tree_depth + 1;
+}
+
+// Launch shape and buffer sizes of one backward pass.
+struct AdjointPlan {
+  uint32_t tw, th, threads, tiles_x, ntiles, groups_x, ngroups, grid;
+  int cap;        // LDS stack entries per lane
+  bool lds_cols;  // the per-lane columns fit in LDS
+  const void* fn;
+  size_t lds, partial_bytes, cols_bytes, spill_bytes;
+};
+
 template <typename R, int K>
-void launchAdjoint(zt_ctx* ctx, DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, const float* ddimg,
-                   R* dgrad) {
+AdjointPlan planAdjoint(const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu, int cap) {
+  AdjointPlan p{};
+  p.tw = tu.tile_w ? tu.tile_w : 8;
+  p.th = tu.tile_h ? tu.tile_h : 8;
+  if (p.tw % 8 || p.th % 8 || p.tw * p.th > 256)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "adjoint tiles are 8x8 groups of pixels: multiples of 8, at most 256 pixels");
+  p.threads = p.tw * p.th;
+  const uint32_t waves = p.threads / 64;
+  p.tiles_x = ((uint32_t)o.width + p.tw - 1) / p.tw;
+  p.ntiles = p.tiles_x * (((uint32_t)o.height + p.th - 1) / p.th);
+  p.groups_x = ((uint32_t)o.width + 7) / 8;
+  p.ngroups = p.groups_x * (((uint32_t)o.height + 7) / 8);
+  p.cap = cap;
+  const size_t np = (size_t)std::max(s.num_params, 1);
+  const size_t col_bytes = (size_t)waves * kColStride * np * sizeof(R);
+  p.lds_cols = col_bytes <= kLdsColBytes;
+  p.fn = p.lds_cols ? (const void*)adjointKernel<R, K, true> : (const void*)adjointKernel<R, K, false>;
+  p.lds = stackBytes(cap, (int)p.threads) + (p.lds_cols ? col_bytes : 0);
+  p.grid = std::min<uint32_t>(p.ntiles, residentBlocks(p.fn, p.threads, p.lds));
+  p.partial_bytes = (size_t)p.ngroups * np * sizeof(double);
+  p.cols_bytes = p.lds_cols ? 0 : (size_t)p.grid * col_bytes;
+  p.spill_bytes = spillBytes((uint64_t)p.grid * p.threads);
+  return p;
+}
+
+// The backward pass into `grad` (num_params doubles, device memory).
+template <typename R, int K>
+void runAdjoint(zt_ctx* ctx, const DevScene& s, const zt_render_options& o, const zt_gpu_tuning& tu,
+                const AdjointPlan& p, const float* ddimg, double* grad, double* partial, void* cols, int* spill) {
   SceneView<R> view = viewOf<R>(s);
   view.cam = cameraInit(s.camera, (double)o.width / (double)o.height);
   applyBvhWidth(view, s, tu);  // the replay must walk the tree the forward render walked
   const Params pr = makeParams(s, o);
-  const uint32_t tw = 8, threads = 64;
-  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
-  const uint32_t tiles_y = (pr.height + (threads / tw) - 1) / (threads / tw);
-  const size_t lds = stackBytes(kLds, (int)threads) + (size_t)s.num_params * sizeof(R);
-  ctx->spill.ensure(spillBytes((uint64_t)tiles_x * tiles_y * threads));
-  hipLaunchKernelGGL((adjointKernel<R, K>), dim3(tiles_x * tiles_y), dim3(threads), lds, ctx->stream, view, pr, ddimg,
-                     s.param_offsets.as<int>(), s.num_params, dgrad, tw, tiles_x, tiles_x * tiles_y,
-                     ctx->spill.as<int>(), ctx->errflag.as<int>());
+  void* args[] = {&view, (void*)&pr, (void*)&ddimg, (void*)&s.param_offsets.p, (void*)&s.num_params, &cols, &partial,
+                  (void*)&p.tw, (void*)&p.th, (void*)&p.tiles_x, (void*)&p.ntiles, (void*)&p.groups_x,
+                  (void*)&p.cap, &spill, (void*)&ctx->errflag.p};
+  ZT_HIP(hipLaunchKernel(p.fn, dim3(p.grid), dim3(p.threads), args, p.lds, ctx->stream));
+  if (s.num_params > 0)
+    hipLaunchKernelGGL(reduceGroups, dim3((uint32_t)s.num_params), dim3(64), 0, ctx->stream, partial, s.num_params,
+                       p.ngroups, grad);
   ZT_HIP(hipGetLastError());
+}
+
+// Dispatch on precision and on the depth class of the observer's lists.
+template <typename F>
+void withAdjoint(bool f64, uint64_t depth, F&& f) {
+  if (f64) {
+    if (depth <= 8) f(double{}, std::integral_constant<int, 8>{});
+    else if (depth <= 16) f(double{}, std::integral_constant<int, 16>{});
+    else f(double{}, std::integral_constant<int, 32>{});
+  } else {
+    if (depth <= 8) f(float{}, std::integral_constant<int, 8>{});
+    else if (depth <= 16) f(float{}, std::integral_constant<int, 16>{});
+    else f(float{}, std::integral_constant<int, 32>{});
+  }
+}
+
+void checkAdjointErr(zt_ctx* ctx) {
+  int e = 0;
+  ZT_HIP(hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ZT_HIP(hipStreamSynchronize(ctx->stream));
+  if (e & 1) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+  if (e & 2) throw Error(ZT_ERR_INTERNAL, "a path visited more distinct materials than the adjoint tracks");
 }
 
 }  // namespace ztg
@@ -339,43 +448,30 @@ int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opt
     const bool f64 = tu.precision == ZT_PRECISION_FP64;
     const size_t npix = (size_t)opts->width * opts->height;
     const float* ddimg = dL_dimage;
-    DevBuf dimg_buf, grad_buf;
+    DevBuf dimg_buf;
     if (!grad_on_device) {
       dimg_buf.alloc(npix * 3 * sizeof(float));
       ZT_HIP(hipMemcpyAsync(dimg_buf.p, dL_dimage, npix * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
       ddimg = dimg_buf.as<float>();
     }
     const int np = std::max(s.num_params, 1);
-    grad_buf.alloc((size_t)np * (f64 ? sizeof(double) : sizeof(float)));
+    ctx->adj_grad.ensure((size_t)np * sizeof(double));
     resetErr(ctx);
     ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    if (f64) hipLaunchKernelGGL(zeroFill<double>, dim3(1), dim3(256), 0, ctx->stream, grad_buf.as<double>(), np);
-    else hipLaunchKernelGGL(zeroFill<float>, dim3(1), dim3(256), 0, ctx->stream, grad_buf.as<float>(), np);
-    const uint64_t d = opts->max_depth;
-    if (f64) {
-      if (d <= 8) launchAdjoint<double, 8>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
-      else if (d <= 16) launchAdjoint<double, 16>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
-      else launchAdjoint<double, 32>(ctx, s, *opts, tu, ddimg, grad_buf.as<double>());
-    } else {
-      if (d <= 8) launchAdjoint<float, 8>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
-      else if (d <= 16) launchAdjoint<float, 16>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
-      else launchAdjoint<float, 32>(ctx, s, *opts, tu, ddimg, grad_buf.as<float>());
-    }
+    withAdjoint(f64, opts->max_depth, [&](auto tag, auto k) {
+      using R = decltype(tag);
+      constexpr int K = decltype(k)::value;
+      const AdjointPlan p = planAdjoint<R, K>(s, *opts, tu, kLds);
+      ctx->adj_partial.ensure(p.partial_bytes);
+      ctx->adj_cols.ensure(p.cols_bytes);
+      ctx->spill.ensure(p.spill_bytes);
+      runAdjoint<R, K>(ctx, s, *opts, tu, p, ddimg, ctx->adj_grad.as<double>(), ctx->adj_partial.as<double>(),
+                       ctx->adj_cols.p, ctx->spill.as<int>());
+    });
     ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<double> gd(np);
-    if (f64) {
-      ZT_HIP(hipMemcpyAsync(gd.data(), grad_buf.p, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    } else {
-      std::vector<float> gf(np);
-      ZT_HIP(hipMemcpyAsync(gf.data(), grad_buf.p, np * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-      ZT_HIP(hipStreamSynchronize(ctx->stream));
-      for (int i = 0; i < np; ++i) gd[i] = gf[i];
-    }
-    int e = 0;
-    ZT_HIP(hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    ZT_HIP(hipStreamSynchronize(ctx->stream));
-    if (e & 1) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
-    if (e & 2) throw Error(ZT_ERR_INTERNAL, "a path visited more distinct materials than the adjoint tracks");
+    ZT_HIP(hipMemcpyAsync(gd.data(), ctx->adj_grad.p, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    checkAdjointErr(ctx);
     float ms = 0;
     ZT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms = ms;

@@ -122,7 +122,7 @@ extern "C" int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_o
     probe.spp = std::min<uint64_t>(opts->spp, probe_spp ? probe_spp : 16);
     const uint64_t pixels = (uint64_t)probe.width * probe.height;
     const size_t elem = precision == ZT_PRECISION_FP64 ? sizeof(double) : sizeof(float);
-    ctx->scratch.ensure(pixels * 3 * elem);
+    ctx->probe_out.ensure(pixels * 3 * elem);
     std::vector<uint32_t> methods;
     if (method == ZT_METHOD_AUTO) methods = {ZT_METHOD_MEGAKERNEL, ZT_METHOD_WAVEFRONT};
     else methods = {method};
@@ -132,11 +132,11 @@ extern "C" int zt_gpu_autotune(zt_ctx* ctx, uint32_t scene_id, const zt_render_o
       resetErr(ctx);
       ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
       if (tu.method == ZT_METHOD_WAVEFRONT) {
-        if (elem == 8) launchWavefront<double>(ctx, s, probe, tu, ctx->scratch.as<double>());
-        else launchWavefront<float>(ctx, s, probe, tu, ctx->scratch.as<float>());
+        if (elem == 8) launchWavefront<double>(ctx, s, probe, tu, ctx->probe_out.as<double>());
+        else launchWavefront<float>(ctx, s, probe, tu, ctx->probe_out.as<float>());
       } else {
-        if (elem == 8) launchMegakernel<double>(ctx, s, probe, tu, ctx->scratch.as<double>());
-        else launchMegakernel<float>(ctx, s, probe, tu, ctx->scratch.as<float>());
+        if (elem == 8) launchMegakernel<double>(ctx, s, probe, tu, ctx->probe_out.as<double>());
+        else launchMegakernel<float>(ctx, s, probe, tu, ctx->probe_out.as<float>());
       }
       ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
       checkErr(ctx);

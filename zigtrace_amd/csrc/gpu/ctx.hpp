@@ -128,11 +128,12 @@ struct zt_ctx {
   std::string err;
   std::vector<std::unique_ptr<ztg::DevScene>> scenes;  // index = scene id; null = released
   zt_gpu_stats stats{};
-  ztg::DevBuf out, errflag, scratch;
+  ztg::DevBuf out, errflag, probe_out;  // probe_out: the autotuner's trial frames
   ztg::DevBuf spill;  // traversal stack overflow, sized per launch (grow-only)
   ztg::DevBuf bands;  // per-XCD tile counters of the persistent megakernel
   ztg::DevBuf jobs;   // scene table of a many-scene wave-scheduled launch
   ztg::DevBuf tile_cost, tile_order;  // longest-first tile order of the wave-scheduled megakernel
+  ztg::DevBuf adj_partial, adj_cols, adj_grad;  // backward: per-group partials, global lane columns, result
   ztg::WfBuffers wf;
   std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };

@@ -474,7 +474,7 @@ void zt_gpu_destroy(zt_ctx* ctx) {
   ctx->scenes.clear();
   ctx->out.reset();
   ctx->errflag.reset();
-  ctx->scratch.reset();
+  ctx->probe_out.reset();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
