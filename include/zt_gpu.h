@@ -117,6 +117,7 @@ int zt_gpu_get_stats(const zt_ctx* ctx, zt_gpu_stats* out);
 int zt_gpu_device_alloc(zt_ctx* ctx, uint64_t bytes, void** out);
 int zt_gpu_device_free(zt_ctx* ctx, void* ptr);
 int zt_gpu_copy_to_host(zt_ctx* ctx, void* host, const void* device, uint64_t bytes);
+int zt_gpu_copy_to_device(zt_ctx* ctx, void* device, const void* host, uint64_t bytes);
 
 /* Scenes ----------------------------------------------------------------- */
 /* Copies a built scene to the device in the GPU layout (64-byte two-child

@@ -119,6 +119,7 @@ _SIGS = {
     "zt_gpu_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "zt_gpu_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "zt_gpu_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "zt_gpu_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "zt_gpu_scene_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "zt_gpu_scene_release": (C.c_int, [C.c_void_p, C.c_uint32]),
     "zt_gpu_render": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),

@@ -522,6 +522,15 @@ int zt_gpu_copy_to_host(zt_ctx* ctx, void* host, const void* device, uint64_t by
   });
 }
 
+int zt_gpu_copy_to_device(zt_ctx* ctx, void* device, const void* host, uint64_t bytes) {
+  if (!ctx || (bytes && (!host || !device))) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    setDevice(ctx);
+    ZT_HIP(hipMemcpyAsync(device, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int zt_gpu_scene_upload(zt_ctx* ctx, const zt_scene_view* v, uint32_t* scene_id) {
   if (!ctx || !scene_id) return ZT_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {

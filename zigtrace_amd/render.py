@@ -118,6 +118,10 @@ class Context:
         self._check(N.lib().zt_gpu_copy_to_host(self._h, N.ptr(out), C.c_void_p(address), out.nbytes))
         return out
 
+    def to_device(self, address: int, data: np.ndarray) -> None:
+        data = np.ascontiguousarray(data)
+        self._check(N.lib().zt_gpu_copy_to_device(self._h, C.c_void_p(address), N.ptr(data), data.nbytes))
+
     def upload(self, scene: Scene) -> int:
         view = scene.view()
         sid = C.c_uint32()
