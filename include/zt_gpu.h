@@ -215,6 +215,22 @@ int zt_gpu_forward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts
 int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts, const zt_gpu_tuning* tuning,
                     const float* params, const float* dL_dimage, int grad_on_device, double* dL_dparams);
 
+/* No counterpart in the reference: zt_gpu_backward for callers that cannot
+ * allocate (code under HIP graph capture, tight training or serving loops,
+ * harnesses that forbid hipMalloc). It calls no allocating routine: the
+ * partial sums live in a fixed 8 MiB device array the library holds on each
+ * GPU (set up by zt_gpu_create), the traversal stack entirely in LDS.
+ * dL_dimage (width*height*3 floats) and dL_dparams (num_params doubles) are
+ * device memory; the parameters are the scene's current ones (set them
+ * beforehand with zt_gpu_set_params). Same kernels and same bits as
+ * zt_gpu_backward. Rejected with ZT_ERR_INVALID_ARGUMENT: ceil(w/8)*ceil(h/8)
+ * * num_params > 1048576, parameters beyond the LDS columns (126 in fp32,
+ * 63 in fp64 for 8x8 tiles), a BVH needing more than 64 stack entries. All
+ * calls on one GPU share the array: calls that overlap on one GPU (other
+ * streams, other contexts, other threads) race. */
+int zt_gpu_backward_noalloc(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts,
+                            const zt_gpu_tuning* tuning, const float* dL_dimage, double* dL_dparams);
+
 #ifdef __cplusplus
 }
 #endif

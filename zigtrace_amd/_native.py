@@ -141,6 +141,7 @@ _SIGS = {
     "zt_gpu_forward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "zt_gpu_backward": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                   C.c_void_p]),
+    "zt_gpu_backward_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     # zt_host.h
     "zt_scene_create": (C.c_void_p, []),
     "zt_scene_destroy": (None, [C.c_void_p]),

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../device/dev_path.hpp"
@@ -379,6 +380,40 @@ void checkAdjointErr(zt_ctx* ctx) {
   if (e & 2) throw Error(ZT_ERR_INTERNAL, "a path visited more distinct materials than the adjoint tracks");
 }
 
+// ---------------------------------------------------------------- allocation-free backward
+// Scratch of zt_gpu_backward_noalloc: the rows of per-group partials (8 MiB,
+// one copy on every GPU that loads the library, held until the process
+// ends). In host code `scratch` is only a stand-in for the symbol: its device
+// address on each GPU comes from hipGetSymbolAddress with that GPU current
+// (initNoAllocScratch, run by zt_gpu_create) and is used everywhere instead.
+static constexpr size_t kNoAllocScratchDoubles = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxScratchDevices];
+
+void initNoAllocScratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxScratchDevices) throw Error(ZT_ERR_INVALID_ARGUMENT, "device index out of range");
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  ZT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_dev[device] = (double*)p;
+}
+
+static double* noAllocScratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  double* p = device >= 0 && device < kMaxScratchDevices ? g_scratch_dev[device] : nullptr;
+  if (!p) throw Error(ZT_ERR_INTERNAL, "no-alloc scratch not set up for this device");
+  return p;
+}
+
+// Clears the launch status words (a kernel, not a memset node, so the call
+// can sit inside a captured graph).
+__global__ void clearStatus(int* status) {
+  if (threadIdx.x < kStatusBytes / sizeof(int)) status[threadIdx.x] = 0;
+}
+
 }  // namespace ztg
 
 using namespace ztg;
@@ -460,12 +495,12 @@ int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opt
     ZT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     withAdjoint(f64, opts->max_depth, [&](auto tag, auto k) {
       using R = decltype(tag);
-      constexpr int K = decltype(k)::value;
-      const AdjointPlan p = planAdjoint<R, K>(s, *opts, tu, kLds);
+      constexpr int kList = decltype(k)::value;
+      const AdjointPlan p = planAdjoint<R, kList>(s, *opts, tu, kLds);
       ctx->adj_partial.ensure(p.partial_bytes);
       ctx->adj_cols.ensure(p.cols_bytes);
       ctx->spill.ensure(p.spill_bytes);
-      runAdjoint<R, K>(ctx, s, *opts, tu, p, ddimg, ctx->adj_grad.as<double>(), ctx->adj_partial.as<double>(),
+      runAdjoint<R, kList>(ctx, s, *opts, tu, p, ddimg, ctx->adj_grad.as<double>(), ctx->adj_partial.as<double>(),
                        ctx->adj_cols.p, ctx->spill.as<int>());
     });
     ZT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -476,6 +511,47 @@ int zt_gpu_backward(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opt
     ZT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms = ms;
     for (int i = 0; i < s.num_params; ++i) dL_dparams[i] = gd[i];
+  });
+}
+
+int zt_gpu_backward_noalloc(zt_ctx* ctx, uint32_t scene_id, const zt_render_options* opts,
+                            const zt_gpu_tuning* tuning, const float* dL_dimage, double* dL_dparams) {
+  if (!ctx) return ZT_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    checkOptions(opts);
+    if (!dL_dimage || !dL_dparams) throw Error(ZT_ERR_INVALID_ARGUMENT, "null buffer");
+    if (opts->max_depth > 32) throw Error(ZT_ERR_INVALID_ARGUMENT, "the adjoint supports max_depth <= 32");
+    const DevScene& s = sceneOf(ctx, scene_id);
+    zt_gpu_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.precision > ZT_PRECISION_FP64) throw Error(ZT_ERR_INVALID_ARGUMENT, "bad precision");
+    setDevice(ctx);
+    double* partial = noAllocScratch(ctx->device);
+    withAdjoint(tu.precision == ZT_PRECISION_FP64, opts->max_depth, [&](auto tag, auto k) {
+      using R = decltype(tag);
+      constexpr int kList = decltype(k)::value;
+      // The whole traversal stack in LDS, sized to the deepest the scene's
+      // tree can need, so no spill buffer exists.
+      SceneView<R> v = viewOf<R>(s);
+      applyBvhWidth(v, s, tu);
+      const int need = stackNeed(s, !K<R>::exact && v.nodes4 != nullptr);
+      if (need > 64)
+        throw Error(ZT_ERR_INVALID_ARGUMENT, "the scene's BVH needs more than the 64 traversal stack entries that "
+                                             "fit in LDS without a spill buffer");
+      const AdjointPlan p = planAdjoint<R, kList>(s, *opts, tu, std::max(8, (need + 1) & ~1));
+      if (!p.lds_cols)
+        throw Error(ZT_ERR_INVALID_ARGUMENT, "too many parameters for the per-lane columns in LDS");
+      if ((size_t)p.ngroups * (size_t)std::max(s.num_params, 1) > kNoAllocScratchDoubles)
+        throw Error(ZT_ERR_INVALID_ARGUMENT, "8x8 pixel groups x parameters exceed the 8 MiB no-alloc scratch "
+                                             "(1048576 partial sums)");
+      hipLaunchKernelGGL(clearStatus, dim3(1), dim3(64), 0, ctx->stream, ctx->errflag.as<int>());
+      runAdjoint<R, kList>(ctx, s, *opts, tu, p, dL_dimage, dL_dparams, partial, nullptr, nullptr);
+    });
+    ZT_HIP(hipMemcpyAsync(ctx->status_host, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZT_HIP(hipStreamSynchronize(ctx->stream));
+    const int e = ctx->status_host[0];
+    if (e & 1) throw Error(ZT_ERR_INTERNAL, "device traversal stack overflow");
+    if (e & 2) throw Error(ZT_ERR_INTERNAL, "a path visited more distinct materials than the adjoint tracks");
   });
 }
 

@@ -135,12 +135,16 @@ struct zt_ctx {
   ztg::DevBuf tile_cost, tile_order;  // longest-first tile order of the wave-scheduled megakernel
   ztg::DevBuf adj_partial, adj_cols, adj_grad;  // backward: per-group partials, global lane columns, result
   ztg::WfBuffers wf;
+  int* status_host = nullptr;  // pinned: where zt_gpu_backward_noalloc reads the status words
   std::map<ztg::TuneKey, ztg::TuneEntry> tuned;
 };
 
 namespace ztg {
 
 DevScene& sceneOf(zt_ctx* ctx, uint32_t id);
+// Resolves the device address of zt_gpu_backward_noalloc's scratch on
+// `device` (the current device), once per device.
+void initNoAllocScratch(int device);
 void setDevice(zt_ctx* ctx);
 
 // Camera.init (camera.zig:29-55) in f64 on the host.

@@ -456,8 +456,11 @@ int zt_gpu_create(int device, zt_ctx** out) {
     ZT_HIP(hipEventCreate(&ctx->ev0));
     ZT_HIP(hipEventCreate(&ctx->ev1));
     ctx->errflag.alloc(ztd::kStatusBytes);
+    ZT_HIP(hipHostMalloc((void**)&ctx->status_host, ztd::kStatusBytes, hipHostMallocDefault));
+    initNoAllocScratch(device);
   });
   if (rc != ZT_OK) {
+    if (ctx->status_host) (void)hipHostFree(ctx->status_host);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -475,6 +478,7 @@ void zt_gpu_destroy(zt_ctx* ctx) {
   ctx->out.reset();
   ctx->errflag.reset();
   ctx->probe_out.reset();
+  if (ctx->status_host) (void)hipHostFree(ctx->status_host);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
