@@ -196,3 +196,57 @@ def test_example_scenes_render_finite():
         s = zt.parse(scene_text(name))
         img = zt.render(s, zt.Options(40, 24, 4, 12))
         assert np.isfinite(img).all() and img.max() > 0, name
+
+
+def _lit_sphere(color):
+    s = zt.Scene()
+    s.background = zt.solid((1, 1, 1))
+    m = s.add_material(zt.lambertian(color))
+    s.add_sphere((0, 0, -3), 1, m)
+    s.camera = zt.CameraConfig(vfov=10)
+    s.build()
+    return s
+
+
+def test_recreated_scene_is_not_served_from_a_stale_device_copy():
+    """zt_render keeps a device copy per scene; a scene created after another was destroyed
+    (often at the same address, with the same mutation count) must render its own data."""
+    import gc
+    for k in range(6):
+        color = (0.1 + 0.1 * k, 0.5, 0.25)
+        s = _lit_sphere(color)
+        img = zt.render(s, opts(width=8, height=8, spp=2))
+        assert np.allclose(img, np.broadcast_to(color, img.shape)), k  # render.zig:230-243: albedo exactly
+        del s
+        gc.collect()
+
+
+def test_scene_rendered_on_several_threads_then_destroyed():
+    """Each host thread renders through its own default context; destroying the scene
+    releases the copies in every context, and a thread's context goes when it exits."""
+    import gc
+    import threading
+    s = _lit_sphere((0.5, 0.25, 0.125))
+    out, errors = {}, []
+
+    def work(i):
+        try:
+            out[i] = zt.render(s, opts(width=8, height=8, spp=2, seed=i))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors and len(out) == 3
+    for img in out.values():
+        assert np.array_equal(img, np.broadcast_to([0.5, 0.25, 0.125], img.shape))
+    del s
+    gc.collect()
+    t = threading.Thread(target=work, args=(7,))
+    s = _lit_sphere((0.2, 0.4, 0.6))
+    t.start()
+    t.join()
+    assert not errors and np.allclose(out[7], np.broadcast_to((0.2, 0.4, 0.6), out[7].shape))

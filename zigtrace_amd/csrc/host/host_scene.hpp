@@ -34,6 +34,7 @@ struct zt_scene {
   std::vector<uint32_t> indices;
   zth::HostLights lights;
   uint64_t version = 1;  // bumped by every mutation; the GPU copy is keyed on it
+  uint64_t uid = 0;      // unique for the process's lifetime (an address may be reused)
 
   zt_scene() {
     camera = zt_camera_config{{0, 0, 0}, {0, 0, -1}, {0, 1, 0}, 60, 0, 0, 0, 0};  // camera.zig:8-18

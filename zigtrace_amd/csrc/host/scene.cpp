@@ -1,11 +1,14 @@
 // Host scene API (scene.zig), Lights.build (light.zig:68-94) and the
 // render.render entry (render.zig:152-180) that hands the built scene to the
 // GPU through zt_gpu.h.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "host_scene.hpp"
@@ -54,18 +57,59 @@ void buildLights(const std::vector<zt_shape>& shapes, const std::vector<zt_mater
 
 namespace {
 
-// Per-thread default GPU context and the device copies of scenes in it.
+// Per-thread default GPU context (zt_render) and the device copies of scenes
+// in it, keyed by the scene's process-unique id and version. Every thread's
+// record is listed in a registry, so destroying a scene releases its copies
+// in every context; a record's mutex is held while its thread renders and
+// while another thread drops copies from it.
+std::atomic<uint64_t> g_next_uid{1};
+
 struct DeviceCopy {
-  const zt_scene* scene;
-  uint64_t version;
+  uint64_t uid, version;
   uint32_t id;
 };
-// The context lives until the process ends: destroying it from a
-// thread_local destructor at exit could run after the HIP runtime's own
-// teardown.
+struct ThreadGpu;
+struct Registry {
+  std::mutex mu;
+  std::vector<ThreadGpu*> all;
+};
+// Never destroyed: threads may exit while the process tears down statics.
+Registry& registry() {
+  static Registry* r = new Registry;
+  return *r;
+}
+// The thread that loaded the library: its context is left to the end of the
+// process (destroying it from a thread_local destructor at exit could run
+// after the HIP runtime's own teardown). Other threads free theirs on exit.
+const std::thread::id g_loader_thread = std::this_thread::get_id();
+
 struct ThreadGpu {
+  std::mutex mu;
   zt_ctx* ctx = nullptr;
   std::vector<DeviceCopy> copies;
+  ThreadGpu() {
+    std::lock_guard<std::mutex> lock(registry().mu);
+    registry().all.push_back(this);
+  }
+  ~ThreadGpu() {
+    {
+      std::lock_guard<std::mutex> lock(registry().mu);
+      auto& v = registry().all;
+      v.erase(std::remove(v.begin(), v.end(), this), v.end());
+    }
+    if (ctx && std::this_thread::get_id() != g_loader_thread) zt_gpu_destroy(ctx);
+  }
+  // Drops the copies of scene `uid` (all versions). Caller holds `mu`.
+  void drop(uint64_t uid) {
+    for (size_t i = 0; i < copies.size();) {
+      if (copies[i].uid == uid) {
+        zt_gpu_scene_release(ctx, copies[i].id);
+        copies.erase(copies.begin() + (long)i);
+      } else {
+        ++i;
+      }
+    }
+  }
 };
 thread_local ThreadGpu g_gpu;
 
@@ -73,16 +117,19 @@ thread_local ThreadGpu g_gpu;
 
 extern "C" {
 
-zt_scene* zt_scene_create(void) { return new (std::nothrow) zt_scene(); }
+zt_scene* zt_scene_create(void) {
+  zt_scene* s = new (std::nothrow) zt_scene();
+  if (s) s->uid = g_next_uid.fetch_add(1);
+  return s;
+}
 
 void zt_scene_destroy(zt_scene* s) {
   if (!s) return;
-  // Drop this thread's device copy of the scene, if any.
-  for (size_t i = 0; i < g_gpu.copies.size(); ++i) {
-    if (g_gpu.copies[i].scene == s) {
-      zt_gpu_scene_release(g_gpu.ctx, g_gpu.copies[i].id);
-      g_gpu.copies.erase(g_gpu.copies.begin() + (long)i);
-      break;
+  {
+    std::lock_guard<std::mutex> lock(registry().mu);
+    for (ThreadGpu* t : registry().all) {
+      std::lock_guard<std::mutex> tl(t->mu);
+      t->drop(s->uid);
     }
   }
   delete s;
@@ -229,6 +276,7 @@ int zt_bvh_build_threads(const zt_shape* shapes, uint64_t n, uint32_t threads, z
 
 int zt_default_context(zt_ctx** out) {
   if (!out) return ZT_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_gpu.mu);
   if (!g_gpu.ctx) {
     const int rc = zt_gpu_create(0, &g_gpu.ctx);
     if (rc != ZT_OK) {
@@ -240,12 +288,16 @@ int zt_default_context(zt_ctx** out) {
   return ZT_OK;
 }
 
-int zt_scene_device_id(zt_scene* s, zt_ctx* ctx, uint32_t* id_out) {
-  if (!s || !ctx || !id_out) return ZT_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace {
+// zt_scene_device_id for the calling thread's default context (caching the
+// copy); caller holds g_gpu.mu.
+int deviceIdLocked(zt_scene* s, zt_ctx* ctx, uint32_t* id_out) {
   if (ctx == g_gpu.ctx) {
     for (size_t i = 0; i < g_gpu.copies.size(); ++i) {
       DeviceCopy& c = g_gpu.copies[i];
-      if (c.scene != s) continue;
+      if (c.uid != s->uid) continue;
       if (c.version == s->version) {
         *id_out = c.id;
         return ZT_OK;
@@ -260,9 +312,18 @@ int zt_scene_device_id(zt_scene* s, zt_ctx* ctx, uint32_t* id_out) {
   uint32_t id = 0;
   const int rc = zt_gpu_scene_upload(ctx, &v, &id);
   if (rc != ZT_OK) return rc;
-  if (ctx == g_gpu.ctx) g_gpu.copies.push_back(DeviceCopy{s, s->version, id});
+  if (ctx == g_gpu.ctx) g_gpu.copies.push_back(DeviceCopy{s->uid, s->version, id});
   *id_out = id;
   return ZT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int zt_scene_device_id(zt_scene* s, zt_ctx* ctx, uint32_t* id_out) {
+  if (!s || !ctx || !id_out) return ZT_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_gpu.mu);
+  return deviceIdLocked(s, ctx, id_out);
 }
 
 int zt_render(zt_scene* s, const zt_render_options* o, const zt_gpu_tuning* tuning, double* pixels) {
@@ -274,8 +335,11 @@ int zt_render(zt_scene* s, const zt_render_options* o, const zt_gpu_tuning* tuni
   zt_ctx* ctx = nullptr;
   int rc = zt_default_context(&ctx);
   if (rc != ZT_OK) return rc;
+  // Held for the whole render: another thread destroying a scene waits
+  // before it releases this context's copies.
+  std::lock_guard<std::mutex> lock(g_gpu.mu);
   uint32_t id = 0;
-  rc = zt_scene_device_id(s, ctx, &id);
+  rc = deviceIdLocked(s, ctx, &id);
   if (rc != ZT_OK) return rc;
   const bool f64 = tuning && tuning->precision == ZT_PRECISION_FP64;
   const size_t n = (size_t)o->width * o->height * 3;
