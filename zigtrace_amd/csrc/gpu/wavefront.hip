@@ -20,9 +20,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../device/dev_path.hpp"
+#include "../device/dev_wave.hpp"
 #include "ctx.hpp"
 #include "launch.hpp"
 #include "wavefront.hpp"
@@ -42,7 +44,9 @@ struct WfShared {
   uint32_t* qext[2];  // extension queues (double-buffered)
   uint32_t* qsh;      // shadow queue
   // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64),
-  // [8..9] extension rays traced (u64), [10..11] shadow rays traced (u64)
+  // [8..9] extension rays traced (u64), [10..11] shadow rays traced (u64),
+  // [12] / [13] read cursors of the queue-fed tracer over the extension /
+  // shadow queue (each cleared by the kernel that runs before that trace)
   uint32_t* counters;
   int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
@@ -173,9 +177,228 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
+// The scenes of a trace launch, for the wave-level traversal passes
+// (dev_wave.hpp): one scene in kernel arguments, or the batch's table.
+struct WfOne {
+  static constexpr bool kSingle = true;
+  SceneView<float> sc;
+  ZT_D const SceneView<float>& view(uint32_t) const { return sc; }
+};
+struct WfMany {
+  static constexpr bool kSingle = false;
+  const WfScene<float>* scenes;
+  ZT_D const SceneView<float>& view(uint32_t si) const { return scenes[si].view; }
+};
+
+// The same queries as wfTrace (fp32, four-wide trees), traced by persistent
+// one-wave workgroups under the megakernel's wave-uniform node and leaf
+// passes: lanes without a ray take the next queue entries (one atomic per
+// wave on the queue's cursor), and the wave stops its passes to take more
+// once `refill` lanes have finished, until the queue is exhausted. (Cutting
+// the queue into eight chunks taken first by the waves of one XCD each was
+// measured 2x slower on the 1080p mesh field.)
+template <bool Q, typename Src>
+ZT_D void wfTraceWaveBody(const Src& src, const WfShared<float>& w, int cur, int shadow, int cap, uint32_t refill) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
+  unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
+  uint8_t* owner = (uint8_t*)(keys + 64);
+  const uint32_t lane = threadIdx.x;
+  const bool any = shadow != 0;
+  const uint32_t n = any ? w.counters[2] : w.counters[cur];
+  if (blockIdx.x == 0 && lane == 0) {
+    *(unsigned long long*)&w.counters[any ? 10 : 8] += n;
+    if (!any) {  // the queues the kernels after this one append to
+      w.counters[1 - cur] = 0;
+      w.counters[2] = 0;
+    }
+  }
+  const uint32_t* q = any ? w.qsh : w.qext[cur];
+  uint32_t* cursor = &w.counters[any ? 13 : 12];
+  bool tracing = false, drained = false;
+  uint32_t slot = 0, si = 0;
+  Query4 qy{};
+  Trav4 tr;
+  int overflow = 0;
+  for (;;) {
+    if (!drained) {
+      const uint64_t need = __ballot(!tracing);
+      if (need) {
+        const uint32_t k = (uint32_t)__popcll(need);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, k);
+        base = __shfl(base, 0);
+        if (base + k >= n) drained = true;
+        const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (!tracing && idx < n) {
+          slot = q[idx];
+          const WfSlot<float>& sl = w.slots[slot];
+          si = sl.scene;
+          qy = Query4{sl.p.ro, any ? sl.p.sh_d : sl.p.rd, (float)kRayEps, sl.p.self, any ? sl.p.sh_excl : -1, any};
+          tr.start(qy, any ? sl.p.sh_tmax : INFINITY);
+          tracing = true;
+        }
+      }
+    }
+    if (__ballot(tracing) == 0) {
+      if (drained) break;
+      continue;
+    }
+    for (;;) {
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      if (!drained && (uint32_t)__popcll(__ballot(!tracing)) >= refill) break;
+      if (wavePass<Q>(src, si, tr, qy, tracing, at_leaf, n_leaf, n_node, stk, &overflow, keys, owner)) {
+        const HitRec<float> h = tr.finish(src.view(si), qy);
+        if (!any) w.hits[slot].x = h.t;
+        w.hits[slot].y = __int_as_float(h.prim);
+        tracing = false;
+      }
+    }
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfTraceWaveOne(
+    WfOne src, WfShared<float> w, int cur, int shadow, int cap, uint32_t refill) {
+  wfTraceWaveBody<Q>(src, w, cur, shadow, cap, refill);
+}
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfTraceWaveMany(
+    WfMany src, WfShared<float> w, int cur, int shadow, int cap, uint32_t refill) {
+  wfTraceWaveBody<Q>(src, w, cur, shadow, cap, refill);
+}
+
+// The tail of a frame: once every pixel has been claimed and the live paths
+// no longer fill the GPU, each remaining slot -- whose next extension ray is
+// pending after a full bounce iteration -- is finished by a persistent
+// wave-scheduled kernel: a lane takes a slot from `list` (the live extension
+// queue), continues its path and its pixel's remaining samples from the
+// slot's state with the same device functions, writes the pixel, and takes
+// the next slot. The bounce loop then stops; each pixel still runs its
+// samples in order from its own stream, so the image is the same bits.
+template <bool S>
+struct DrainSrc {
+  static constexpr bool kSingle = S;
+  const WfScene<float>* scenes;
+  ZT_D const WfScene<float>& scene(uint32_t si) const { return scenes[S ? 0 : si]; }
+  ZT_D const SceneView<float>& view(uint32_t si) const { return scene(si).view; }
+};
+
+template <bool Q, bool kSingle>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfDrain(
+    WfShared<float> w, uint32_t n, int cap, uint32_t refill) {
+  constexpr int kResolve = 4;  // a traversal result is waiting to be applied
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
+  unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
+  uint8_t* owner = (uint8_t*)(keys + 64);
+  const DrainSrc<kSingle> src{w.scenes};
+  const uint32_t* list = w.qext[0];
+  uint32_t* cursor = &w.counters[12];
+  const uint32_t lane = threadIdx.x;
+  bool done = false;  // the list is used up (wave-uniform)
+  bool has = false, tracing = false, sh = false;
+  uint32_t si = 0, pixel = 0, sample = 0;
+  int phase = kExtend;
+  Rng rng{};
+  V3<float> sum = splat(0.0f);
+  Path<float> st;
+  Trav4 tr;
+  int overflow = 0;
+  uint64_t n_ext = 0, n_sh = 0;
+  for (;;) {
+    if (!done) {
+      const uint64_t need = __ballot(!has);
+      if (need) {
+        const uint32_t k = (uint32_t)__popcll(need);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, k);
+        base = __shfl(base, 0);
+        if (base + k >= n) done = true;
+        const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (!has && idx < n) {
+          const WfSlot<float>& sl = w.slots[list[idx]];
+          si = sl.scene;
+          pixel = sl.pixel;
+          sample = sl.sample;
+          rng = sl.rng;
+          sum = sl.sum;
+          st = sl.p;
+          phase = kExtend;
+          has = true;
+        }
+      }
+    }
+    const bool idle = has && !tracing;
+    if (idle) {
+      const WfScene<float>& sc = src.scene(si);
+      for (;;) {
+        if (phase == kResolve) {
+          const HitRec<float> hit{tr.closest, tr.best};
+          phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc.view, sc.pr, hit, rng, st);
+        }
+        if (phase == kFinish) {
+          if (isFinite(st.radiance)) sum = add(sum, st.radiance);  // render.zig:141-142
+          ++sample;
+          phase = kCamera;
+        }
+        if (phase == kCamera) {
+          if (sample == sc.pr.spp) {
+            const float inv_spp = 1.0f / (float)sc.pr.spp;
+            sc.out[3 * (uint64_t)pixel + 0] = sum.x * inv_spp;
+            sc.out[3 * (uint64_t)pixel + 1] = sum.y * inv_spp;
+            sc.out[3 * (uint64_t)pixel + 2] = sum.z * inv_spp;
+            has = false;
+            break;
+          }
+          phase = beginPath<float>(sc.view, sc.pr, pixel % sc.pr.width, pixel / sc.pr.width, rng, st);
+          continue;  // max_depth > 0 here: an extension ray follows
+        }
+        sh = phase == kShadow;
+        tr.start(Query4{st.ro, sh ? st.sh_d : st.rd, (float)kRayEps, st.self, sh ? st.sh_excl : -1, sh},
+                 sh ? st.sh_tmax : INFINITY);
+        tracing = true;
+        break;
+      }
+    }
+    {
+      const uint64_t started = __ballot(idle && tracing), shadow = __ballot(idle && tracing && sh);
+      n_ext += (uint64_t)__popcll(started & ~shadow);
+      n_sh += (uint64_t)__popcll(shadow);
+    }
+    if (__ballot(tracing) == 0) {
+      if (done) break;
+      continue;
+    }
+    for (;;) {
+      const Query4 q{st.ro, sh ? st.sh_d : st.rd, (float)kRayEps, st.self, sh ? st.sh_excl : -1, sh};
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      if ((uint32_t)__popcll(__ballot(!tracing && (has || !done))) >= refill) break;
+      if (wavePass<Q>(src, si, tr, q, tracing, at_leaf, n_leaf, n_node, stk, &overflow, keys, owner)) {
+        (void)tr.finish(src.view(si), q);
+        tracing = false;
+        phase = kResolve;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
+    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
   const uint32_t n = w.counters[cur];
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[13] = 0;  // the shadow trace's cursor
   const uint32_t stride = gridDim.x * blockDim.x;
   const int nxt = 1 - cur;
   // Uniform trip count across the wave so every lane reaches the appends.
@@ -210,6 +433,7 @@ __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
 template <typename R>
 __global__ __launch_bounds__(256) void wfResolve(WfShared<R> w, int nxt) {
   const uint32_t n = w.counters[2];
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[12] = 0;  // the next extension trace's cursor
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
   for (uint32_t b = base0; b < n; b += stride) {
@@ -267,7 +491,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   b.qext0.ensure((size_t)nslots * 4);
   b.qext1.ensure((size_t)nslots * 4);
   b.qsh.ensure((size_t)nslots * 4);
-  b.counters.ensure(64);
+  b.counters.ensure(16 * sizeof(uint32_t));
   ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
                         ctx->stream));
 
@@ -284,15 +508,65 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.num_slots = nslots;
   w.total_pixels = total;
 
-  const int cap = tu.stack_entries ? (int)tu.stack_entries : kLds;
+  // fp32 scenes that all have four-wide trees are traced by the queue-fed
+  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead).
+  bool wave_trace = false, quant = false;
+  if constexpr (!K<R>::exact) {
+    wave_trace = tu.schedule != 1;
+    quant = scenes[0].view.nodes4q != nullptr;
+    for (const auto& sc : scenes) {
+      wave_trace = wave_trace && sc.view.nodes4 != nullptr;
+      if ((sc.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
+    }
+  }
+  const int cap = tu.stack_entries ? (int)tu.stack_entries : (wave_trace ? 12 : kLds);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  const uint32_t refill = tu.refill ? tu.refill : 32;
+  if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
   const size_t lds = stackBytes(cap, (int)threads);
-  const int g_tr = gridFor<R>((const void*)wfTrace<R>, threads, lds);
+  const bool single = scenes.size() == 1;
+  const void* wfn = nullptr;
+  size_t wlds = 0;
+  int g_wave = 0;
+  if (wave_trace) {
+    wlds = stackBytes(cap, 64) + kLeafPassBytes;
+    wfn = single ? (quant ? (const void*)wfTraceWaveOne<true> : (const void*)wfTraceWaveOne<false>)
+                 : (quant ? (const void*)wfTraceWaveMany<true> : (const void*)wfTraceWaveMany<false>);
+    g_wave = (int)residentBlocks(wfn, 64, wlds);
+  }
+  // The tail hand-off (wfDrain): once every pixel is claimed and no more
+  // live paths remain than the drain kernel holds lanes.
+  const void* dfn = nullptr;
+  int g_drain = 0;
+  if (wave_trace) {
+    dfn = single ? (quant ? (const void*)wfDrain<true, true> : (const void*)wfDrain<false, true>)
+                 : (quant ? (const void*)wfDrain<true, false> : (const void*)wfDrain<false, false>);
+    g_drain = (int)residentBlocks(dfn, 64, wlds);
+  }
+  const uint64_t drain_at = (uint64_t)g_drain * 64;
+  const int g_tr = wave_trace ? 0 : gridFor<R>((const void*)wfTrace<R>, threads, lds);
   const int g_shade = gridFor<R>((const void*)wfShade<R>, threads, 0);
   const int g_res = gridFor<R>((const void*)wfResolve<R>, threads, 0);
   const int g_init = std::max(1, (int)std::min<uint64_t>((nslots + threads - 1) / threads, 65535));
-  ctx->spill.ensure(spillBytes((uint64_t)g_tr * threads));
+  ctx->spill.ensure(spillBytes(std::max<uint64_t>({(uint64_t)g_tr * threads, (uint64_t)g_wave * 64,
+                                                    (uint64_t)g_drain * 64})));
   w.spill = ctx->spill.as<int>();
+  WfOne one{};
+  WfMany many{};
+  if constexpr (!K<R>::exact) {
+    one.sc = scenes[0].view;
+    many.scenes = w.scenes;
+  }
+  auto trace = [&](int cur, int shadow) {
+    if (wave_trace) {
+      int c = cap;
+      uint32_t r = refill;
+      void* args[] = {single ? (void*)&one : (void*)&many, (void*)&w, &cur, &shadow, &c, &r};
+      ZT_HIP(hipLaunchKernel(wfn, dim3(g_wave), dim3(64), args, wlds, ctx->stream));
+    } else {
+      hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, shadow, cap);
+    }
+  };
 
   hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, ctx->stream, w.counters, (uint64_t)nslots);
   hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, ctx->stream, w);
@@ -306,9 +580,9 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   for (int j = 0; j < kBatch; ++j) {
     const int cur = j & 1;
-    hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, 0, cap);
+    trace(cur, 0);
     hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
-    hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, 1, cap);
+    trace(cur, 1);
     hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, ctx->stream, w, 1 - cur);
   }
   const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
@@ -329,18 +603,34 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     max_batches = std::max<uint64_t>(max_batches, per_pixel * rounds / kBatch + 2);
   }
   hipError_t le = hipSuccess;
+  bool drain = false;
   while (active) {
     le = hipGraphLaunch(exec, ctx->stream);
     if (le != hipSuccess) break;
     ++batches;
-    le = hipMemcpyAsync(&active, w.counters, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    uint32_t c[8];  // live extension queue (after an even number of bounces) and the next unclaimed pixel
+    le = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream);
     if (le != hipSuccess) break;
     le = hipStreamSynchronize(ctx->stream);
     if (le != hipSuccess) break;
+    active = c[0];
+    uint64_t next_pixel = 0;
+    std::memcpy(&next_pixel, c + 6, 8);
+    if (dfn && active && next_pixel >= total && active <= drain_at) {
+      drain = true;
+      break;
+    }
     if (batches > max_batches) {
       le = hipErrorUnknown;
       break;
     }
+  }
+  if (drain) {  // the last resolve cleared the cursor (counters[12]) the drain takes slots with
+    int c = cap;
+    uint32_t r = refill, n = active;
+    void* args[] = {(void*)&w, &n, &c, &r};
+    le = hipLaunchKernel(dfn, dim3((uint32_t)std::min<uint64_t>(g_drain, ((uint64_t)n + 63) / 64)), dim3(64), args,
+                         wlds, ctx->stream);
   }
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
@@ -348,7 +638,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
-  ctx->stats.launches += batches * kBatch * 4 + 1;
+  ctx->stats.launches += batches * kBatch * 4 + 1 + (drain ? 1 : 0);
   hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, ctx->stream, w.counters, ctx->errflag.as<int>());
   ZT_HIP(hipGetLastError());
 }
