@@ -103,6 +103,19 @@ struct DevScene {
 // Grow-only buffers of the wavefront renderer, reused across calls.
 struct WfBuffers {
   DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, counters;
+  // The instantiated bounce-loop graph and the bytes of every argument it
+  // was captured with: reused while they match, captured again otherwise.
+  std::vector<unsigned char> graph_key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void dropGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
+  ~WfBuffers() { dropGraph(); }
 };
 
 // Autotuner record: the fastest configuration of one method for one class of

@@ -35,10 +35,100 @@ namespace ztg {
 
 static constexpr double kRayEps = 1e-4;  // render.zig:26
 
+// Path slots in device memory as a structure of arrays of 16-byte groups
+// (32-byte in f64), so that the lanes of a wave touch one contiguous run per
+// group instead of 64 records 176 bytes apart. Kernels load a slot into a
+// WfSlot (registers), run the device functions on it and store it back.
+template <typename R>
+struct V4T;
+template <>
+struct V4T<float> {
+  using type = float4;
+};
+template <>
+struct V4T<double> {
+  using type = double4;
+};
+template <typename R>
+using V4 = typename V4T<R>::type;
+
+template <typename R>
+struct WfSlots {
+  V4<R>* ro;     // ray origin, prev_pdf
+  V4<R>* rd;     // extension direction
+  V4<R>* shd;    // shadow direction, shadow t_max
+  V4<R>* rad;    // path radiance
+  V4<R>* thr;    // path throughput
+  V4<R>* shc;    // pending next-event contribution
+  V4<R>* sum;    // pixel accumulator
+  int4* meta;    // self, sh_excl, depth, terminated
+  uint4* where;  // scene (0xFFFFFFFF: idle), pixel, sample, -
+  uint4* rng01;  // xoshiro256++ state words 0 and 1
+  uint4* rng23;  // words 2 and 3
+  static constexpr size_t kBytes = 7 * sizeof(V4<R>) + 4 * sizeof(uint4);  // per slot
+};
+
+template <typename R>
+ZT_D V3<R> xyz(const V4<R>& v) {
+  return mk(v.x, v.y, v.z);
+}
+template <typename R>
+ZT_D V4<R> pack4(V3<R> a, R w) {
+  V4<R> r;
+  r.x = a.x;
+  r.y = a.y;
+  r.z = a.z;
+  r.w = w;
+  return r;
+}
+ZT_D uint4 packRng(uint64_t a, uint64_t b) {
+  return make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+}
+ZT_D uint64_t lo64(uint4 v) { return (uint64_t)v.x | ((uint64_t)v.y << 32); }
+ZT_D uint64_t hi64(uint4 v) { return (uint64_t)v.z | ((uint64_t)v.w << 32); }
+
+template <typename R>
+ZT_D void loadSlot(const WfSlots<R>& S, uint32_t i, WfSlot<R>& sl) {
+  const V4<R> a = S.ro[i], b = S.rd[i], c = S.shd[i], d = S.rad[i], e = S.thr[i], f = S.shc[i], g = S.sum[i];
+  const int4 m = S.meta[i];
+  const uint4 wh = S.where[i], r01 = S.rng01[i], r23 = S.rng23[i];
+  sl.p.ro = xyz<R>(a);
+  sl.p.prev_pdf = a.w;
+  sl.p.rd = xyz<R>(b);
+  sl.p.sh_d = xyz<R>(c);
+  sl.p.sh_tmax = c.w;
+  sl.p.radiance = xyz<R>(d);
+  sl.p.throughput = xyz<R>(e);
+  sl.p.sh_contrib = xyz<R>(f);
+  sl.sum = xyz<R>(g);
+  sl.p.self = m.x;
+  sl.p.sh_excl = m.y;
+  sl.p.depth = m.z;
+  sl.p.terminated = m.w;
+  sl.scene = wh.x;
+  sl.pixel = wh.y;
+  sl.sample = wh.z;
+  sl.rng = Rng{lo64(r01), hi64(r01), lo64(r23), hi64(r23)};
+}
+template <typename R>
+ZT_D void storeSlot(const WfSlots<R>& S, uint32_t i, const WfSlot<R>& sl) {
+  S.ro[i] = pack4<R>(sl.p.ro, sl.p.prev_pdf);
+  S.rd[i] = pack4<R>(sl.p.rd, R(0));
+  S.shd[i] = pack4<R>(sl.p.sh_d, sl.p.sh_tmax);
+  S.rad[i] = pack4<R>(sl.p.radiance, R(0));
+  S.thr[i] = pack4<R>(sl.p.throughput, R(0));
+  S.shc[i] = pack4<R>(sl.p.sh_contrib, R(0));
+  S.sum[i] = pack4<R>(sl.sum, R(0));
+  S.meta[i] = make_int4(sl.p.self, sl.p.sh_excl, sl.p.depth, sl.p.terminated);
+  S.where[i] = make_uint4(sl.scene, sl.pixel, sl.sample, 0u);
+  S.rng01[i] = packRng(sl.rng.s0, sl.rng.s1);
+  S.rng23[i] = packRng(sl.rng.s2, sl.rng.s3);
+}
+
 template <typename R>
 struct WfShared {
   WfScene<R>* scenes;
-  WfSlot<R>* slots;
+  WfSlots<R> slots;
   float2* hits;       // per slot: (t, prim bits) of the last extension query
   double* hits_d;     // per slot: t in f64 for the fp64 path
   uint32_t* qext[2];  // extension queues (double-buffered)
@@ -134,11 +224,11 @@ template <typename R>
 __global__ void wfInit(WfShared<R> w) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_slots; s += stride) {
-    WfSlot<R> sl;
+    WfSlot<R> sl{};
     bool go = false;
     sl.scene = 0xFFFFFFFFu;
     if (s < w.total_pixels) go = startPixel(w, sl, s) || claimPixel(w, sl);
-    w.slots[s] = sl;
+    storeSlot(w.slots, s, sl);
     append(w.qext[0], &w.counters[0], s, go);
   }
 }
@@ -161,12 +251,16 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
   int overflow = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t slot = q[i];
-    const WfSlot<R>& sl = w.slots[slot];
-    const SceneView<R>& v = w.scenes[sl.scene].view;
-    const HitRec<R> h = shadow ? query<R>(v, stk, sl.p.ro, sl.p.sh_d, R(kRayEps), sl.p.sh_tmax, sl.p.self,
-                                          sl.p.sh_excl, true, &overflow)
-                               : query<R>(v, stk, sl.p.ro, sl.p.rd, R(kRayEps), R(INFINITY), sl.p.self, -1, false,
-                                          &overflow);
+    const SceneView<R>& v = w.scenes[w.slots.where[slot].x].view;
+    const V3<R> ro = xyz<R>(w.slots.ro[slot]);
+    const int4 m = w.slots.meta[slot];
+    HitRec<R> h;
+    if (shadow) {
+      const V4<R> c = w.slots.shd[slot];
+      h = query<R>(v, stk, ro, xyz<R>(c), R(kRayEps), c.w, m.x, m.y, true, &overflow);
+    } else {
+      h = query<R>(v, stk, ro, xyz<R>(w.slots.rd[slot]), R(kRayEps), R(INFINITY), m.x, -1, false, &overflow);
+    }
     if constexpr (K<R>::exact) {
       if (!shadow) w.hits_d[slot] = h.t;
     } else {
@@ -232,10 +326,11 @@ ZT_D void wfTraceWaveBody(const Src& src, const WfShared<float>& w, int cur, int
         const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
         if (!tracing && idx < n) {
           slot = q[idx];
-          const WfSlot<float>& sl = w.slots[slot];
-          si = sl.scene;
-          qy = Query4{sl.p.ro, any ? sl.p.sh_d : sl.p.rd, (float)kRayEps, sl.p.self, any ? sl.p.sh_excl : -1, any};
-          tr.start(qy, any ? sl.p.sh_tmax : INFINITY);
+          si = w.slots.where[slot].x;
+          const float4 o4 = w.slots.ro[slot], d4 = any ? w.slots.shd[slot] : w.slots.rd[slot];
+          const int4 m = w.slots.meta[slot];
+          qy = Query4{xyz<float>(o4), xyz<float>(d4), (float)kRayEps, m.x, any ? m.y : -1, any};
+          tr.start(qy, any ? d4.w : INFINITY);
           tracing = true;
         }
       }
@@ -321,7 +416,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         if (base + k >= n) done = true;
         const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
         if (!has && idx < n) {
-          const WfSlot<float>& sl = w.slots[list[idx]];
+          WfSlot<float> sl;
+          loadSlot(w.slots, list[idx], sl);
           si = sl.scene;
           pixel = sl.pixel;
           sample = sl.sample;
@@ -410,7 +506,8 @@ __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
     bool to_sh = false, to_ext = false;
     if (active) {
       s = w.qext[cur][i];
-      WfSlot<R> sl = w.slots[s];
+      WfSlot<R> sl;
+      loadSlot(w.slots, s, sl);
       const WfScene<R>& sc = w.scenes[sl.scene];
       HitRec<R> h;
       if constexpr (K<R>::exact) h.t = w.hits_d[s];
@@ -421,7 +518,7 @@ __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
       else if (ph == kExtend) to_ext = true;
       else to_ext = finishPath(w, sl);
       if (ph == kShadow && !sl.p.terminated) to_ext = true;
-      w.slots[s] = sl;
+      storeSlot(w.slots, s, sl);
     }
     append(w.qsh, &w.counters[2], s, to_sh);
     append(w.qext[nxt], &w.counters[nxt], s, to_ext);
@@ -442,10 +539,21 @@ __global__ __launch_bounds__(256) void wfResolve(WfShared<R> w, int nxt) {
     bool to_ext = false;
     if (i < n) {
       s = w.qsh[i];
-      WfSlot<R> sl = w.slots[s];
       const HitRec<R> h{R(0), __float_as_int(w.hits[s].y)};
-      if (afterShadow<R>(h, sl.p) == kFinish) to_ext = finishPath(w, sl);
-      w.slots[s] = sl;
+      if (w.slots.meta[s].w) {  // the path ends at this bounce: the whole slot moves on
+        WfSlot<R> sl;
+        loadSlot(w.slots, s, sl);
+        (void)afterShadow<R>(h, sl.p);
+        to_ext = finishPath(w, sl);
+        storeSlot(w.slots, s, sl);
+      } else if (h.prim < 0) {  // only the radiance changes (afterShadow, render.zig:51)
+        Path<R> p;
+        p.radiance = xyz<R>(w.slots.rad[s]);
+        p.sh_contrib = xyz<R>(w.slots.shc[s]);
+        p.terminated = 0;
+        (void)afterShadow<R>(h, p);
+        w.slots.rad[s] = pack4<R>(p.radiance, R(0));
+      }
     }
     append(w.qext[nxt], &w.counters[nxt], s, to_ext);
   }
@@ -483,9 +591,8 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   const uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
 
   WfBuffers& b = ctx->wf;
-  const size_t slot_bytes = sizeof(WfSlot<R>);
   b.scenes.ensure(scenes.size() * sizeof(WfScene<R>));
-  b.slots.ensure((size_t)nslots * slot_bytes);
+  b.slots.ensure((size_t)nslots * WfSlots<R>::kBytes + 11 * 256);
   b.hits.ensure((size_t)nslots * sizeof(float2));
   b.hits_d.ensure((size_t)nslots * sizeof(double));
   b.qext0.ensure((size_t)nslots * 4);
@@ -495,9 +602,29 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
                         ctx->stream));
 
-  WfShared<R> w{};
+  WfShared<R> w;
+  std::memset((void*)&w, 0, sizeof w);  // padding included: its bytes key the cached graph
   w.scenes = b.scenes.as<WfScene<R>>();
-  w.slots = b.slots.as<WfSlot<R>>();
+  {
+    char* base = (char*)b.slots.p;
+    size_t off = 0;
+    auto carve = [&](size_t elem) {
+      void* p = base + off;
+      off += ((size_t)nslots * elem + 255) & ~(size_t)255;
+      return p;
+    };
+    w.slots.ro = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.rd = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.shd = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.rad = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.thr = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.shc = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.sum = (V4<R>*)carve(sizeof(V4<R>));
+    w.slots.meta = (int4*)carve(sizeof(int4));
+    w.slots.where = (uint4*)carve(sizeof(uint4));
+    w.slots.rng01 = (uint4*)carve(sizeof(uint4));
+    w.slots.rng23 = (uint4*)carve(sizeof(uint4));
+  }
   w.hits = b.hits.as<float2>();
   w.hits_d = b.hits_d.as<double>();
   w.qext[0] = b.qext0.as<uint32_t>();
@@ -551,8 +678,10 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   ctx->spill.ensure(spillBytes(std::max<uint64_t>({(uint64_t)g_tr * threads, (uint64_t)g_wave * 64,
                                                     (uint64_t)g_drain * 64})));
   w.spill = ctx->spill.as<int>();
-  WfOne one{};
-  WfMany many{};
+  WfOne one;
+  WfMany many;
+  std::memset((void*)&one, 0, sizeof one);  // their bytes key the cached graph too
+  std::memset((void*)&many, 0, sizeof many);
   if constexpr (!K<R>::exact) {
     one.sc = scenes[0].view;
     many.scenes = w.scenes;
@@ -573,25 +702,46 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   ZT_HIP(hipGetLastError());
   ctx->stats.launches += 2;
 
-  // Bounce iterations in batches of an even count, captured once as a graph.
+  // Bounce iterations in batches of an even count, captured as a graph. The
+  // context keeps the instantiated graph with the bytes of everything its
+  // launches were captured with, and reuses it while those match (repeated
+  // renders of one scene and size).
   const int kBatch = 16;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  for (int j = 0; j < kBatch; ++j) {
-    const int cur = j & 1;
-    trace(cur, 0);
-    hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
-    trace(cur, 1);
-    hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, ctx->stream, w, 1 - cur);
+  std::vector<unsigned char> key;
+  auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const unsigned char*)p, (const unsigned char*)p + n); };
+  {
+    const size_t elem = sizeof(R);
+    const uint64_t ints[] = {elem, threads, (uint64_t)g_shade, (uint64_t)g_res, (uint64_t)g_tr, (uint64_t)g_wave,
+                             lds, wlds, (uint64_t)cap, refill, (uint64_t)wave_trace, (uint64_t)single};
+    put(ints, sizeof ints);
+    put(&w, sizeof w);
+    put(&wfn, sizeof wfn);
+    if (wave_trace) put(single ? (const void*)&one : (const void*)&many, single ? sizeof one : sizeof many);
   }
-  const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-  if (ce != hipSuccess) throw Error(ZT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (ie != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    throw Error(ZT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+  if (b.graph_key != key) {
+    b.dropGraph();
+    hipGraph_t graph = nullptr;
+    ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < kBatch; ++j) {
+      const int cur = j & 1;
+      trace(cur, 0);
+      hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
+      trace(cur, 1);
+      hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, ctx->stream, w, 1 - cur);
+    }
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (ce != hipSuccess) throw Error(ZT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      throw Error(ZT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    b.graph = graph;
+    b.exec = ex;
+    b.graph_key = std::move(key);
   }
+  hipGraphExec_t exec = b.exec;
   uint32_t active = 1;
   uint64_t batches = 0;
   // Each batch advances every live path by kBatch bounces; a frame needs at
@@ -632,8 +782,6 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     le = hipLaunchKernel(dfn, dim3((uint32_t)std::min<uint64_t>(g_drain, ((uint64_t)n + 63) / 64)), dim3(64), args,
                          wlds, ctx->stream);
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   if (le != hipSuccess) {
     if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
