@@ -88,14 +88,24 @@ def cpu_baseline(workload, target_seconds: float = 20.0):
         rate = int(ctr["paths"]) / max(dt, 1e-9)
         want_paths = rate * target_seconds
         stride = max(1, int(npix * o.spp / max(want_paths, 1)))
+    runs = 1
+    if stride == 1 and dt < target_seconds / 2:
+        # The whole frame is quick: the median of at least 5 more runs (the first was the warm-up).
+        times = []
+        while len(times) < 5 or (sum(times) < target_seconds / 2 and len(times) < 200):
+            t = time.time()
+            ref.render(o.width, o.height, o.spp, o.max_depth, threads=threads, seed=o.seed)
+            times.append(time.time() - t)
+        dt, runs = float(sorted(times)[len(times) // 2]), len(times)
     paths = int(ctr["paths"])
     rays = int(ctr["extend_rays"]) + int(ctr["shadow_rays"])
     return {
         "value": round(paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "hardware_threads": hw,
         "kind": "port",
         "sample": f"{'every pixel' if stride == 1 else f'one pixel in {stride}'} of the {o.width}x{o.height} frame, "
-                  f"all {o.spp} spp ({paths} paths, {dt:.1f}s), seed {o.seed}, C++ f64 restatement of the reference "
-                  "CPU path",
+                  f"all {o.spp} spp ({paths} paths, {dt:.3f}s" + (f", median of {runs} runs after a warm-up" if runs > 1
+                                                                   else "") +
+                  f"), seed {o.seed}, C++ f64 restatement of the reference CPU path",
     }, ctr, rays / max(paths, 1), img, stride
 
 
