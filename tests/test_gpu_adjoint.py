@@ -6,6 +6,11 @@
     numbers: exactly (1e-3) when no Russian roulette can occur (depth <= 3),
     and within 1% in gradient norm at depth 8 once the roulette decision
     flips that FD picks up are averaged out (256x256, 4096 spp, h = 1e-2);
+  * the same at the step h = 1e-3: an fp32 forward cannot resolve a change of
+    1e-3 (its accumulation noise divided by 2h swamps the difference), so the
+    differences come from the fp64 forward, and the roulette flips, whose FD
+    noise grows as 1/sqrt(paths * h), are averaged over 16384 spp on the
+    emitter and one channel of each material;
 and the Adam driver on the differentiable box."""
 import numpy as np
 import pytest
@@ -141,6 +146,30 @@ def test_adjoint_within_1pct_of_central_fd_at_depth_8(gpu_ctx):
     dr.release()
     rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
     assert rel < 0.01, rel
+
+
+def test_adjoint_within_1pct_of_fp64_central_fd_at_h_1e3_depth_8(gpu_ctx):
+    w = W.diff_box(width=256, height=256, spp=16384, max_depth=8)
+    dr = DiffRenderer(gpu_ctx, w.scene, w.options)
+    f64 = DiffRenderer(gpu_ctx, w.scene, w.options, precision=N.PRECISION_FP64)
+    p = box_params(dr)
+    dimg = np.ones((256, 256, 3))
+    g = dr.backward(p, dimg.astype(np.float32))
+    emitter = p.size - 1
+    idx = [3 * m + m % 3 for m in range(emitter // 3)] + [emitter]
+    fd = np.zeros(len(idx))
+    for j, i in enumerate(idx):
+        up, dn = p.copy(), p.copy()
+        up[i] = np.float32(p[i] + 1e-3)
+        dn[i] = np.float32(p[i] - 1e-3)
+        step = float(up[i]) - float(dn[i])  # the parameters are fp32: divide by the step taken
+        fd[j] = (np.sum(dimg * f64.forward(up)) - np.sum(dimg * f64.forward(dn))) / step
+    f64.release()
+    dr.release()
+    rel = np.linalg.norm(g[idx] - fd) / np.linalg.norm(fd)
+    assert rel < 0.01, rel
+    # The emitter cannot flip a decision (light selection ignores Le): exact up to fp32.
+    assert abs(g[emitter] - fd[-1]) < 2e-3 * abs(fd[-1])
 
 
 def test_inverse_rendering_recovers_parameters(gpu_ctx):
