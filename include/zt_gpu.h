@@ -73,13 +73,19 @@ typedef struct zt_gpu_tuning {
   uint32_t variant;       /* megakernel build: 1 = registers unconstrained, 2/3/4 = compiled for >= 4/5/6 waves
                              per SIMD; 0 = default (wave-scheduled: 2; one pixel per lane: fp32 4, fp64 1) */
   uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64 (0 = wave-scheduled: 12;
-                             one pixel per lane: fp32 8, fp64 16; wavefront: 16) */
+                             one pixel per lane: fp32 8, fp64 16; wavefront: 12 for its wave-scheduled
+                             tracer, else 16) */
   uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
                              nodes (the fp64 path always walks the two-child nodes) */
   uint32_t schedule;      /* megakernel: 1 = one pixel per lane, every lane's ray finished before any lane shades;
                              2 = wave-scheduled (fp32 over four-wide nodes): wave-uniform node / leaf passes,
                              lanes shade and take new rays or pixels while other lanes keep traversing;
-                             0 = 2 where it applies, else 1 */
+                             0 = 2 where it applies, else 1.
+                             wavefront: 1 = one ray per lane; 2 = the queue-fed tracer under the same
+                             wave-uniform passes (fp32 over four-wide nodes), with shading and shadow
+                             resolution in kernels of their own (four launches per bounce); 3 = that tracer
+                             shading each hit itself (two launches per bounce); 0 = 2 where it applies,
+                             else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
                              1..64 (0 = 32) */
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes

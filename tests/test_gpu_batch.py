@@ -12,13 +12,13 @@ from zigtrace_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("queue", [0, 4096])
-def test_batch_equals_individual_renders(gpu_ctx, queue):
+@pytest.mark.parametrize("queue,schedule", [(0, 0), (4096, 0), (0, 3), (4500, 3), (700, 1)])
+def test_batch_equals_individual_renders(gpu_ctx, queue, schedule):
     ws = W.batch_scenes(count=10, seed=3, min_prims=50, max_prims=5000, resolutions=(16, 32, 48), spps=(1, 2, 4),
                         depth_range=(2, 12))
     ids = [gpu_ctx.upload(w.scene) for w in ws]
     opts = [w.options for w in ws]
-    outs = gpu_ctx.render_batch(ids, opts, zt.Tuning(method=N.METHOD_WAVEFRONT, queue_paths=queue))
+    outs = gpu_ctx.render_batch(ids, opts, zt.Tuning(method=N.METHOD_WAVEFRONT, queue_paths=queue, schedule=schedule))
     for sid, o, img in zip(ids, opts, outs):
         single = gpu_ctx.render(sid, o, zt.Tuning(method=N.METHOD_MEGAKERNEL))
         assert np.array_equal(single, img)

@@ -1,5 +1,8 @@
 """The wavefront renderer produces the megakernel's image bit for bit, for
-any path-pool size (slot reuse across pixels), block size and precision."""
+any path-pool size (slot reuse across pixels, queues compacted over one or
+many ranges of 4096 slots), block size, schedule (one ray per lane, the
+queue-fed wave tracer with separate shading, the tracer shading each hit
+itself) and precision."""
 import ctypes as C
 
 import numpy as np
@@ -24,9 +27,12 @@ def test_wavefront_equals_megakernel(name, w, h, spp, depth, precision):
     s = zt.parse(scene_text(name))
     o = zt.Options(w, h, spp, depth, seed=5)
     mega = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision))
-    for block, queue in [(256, 0), (64, 64), (128, 1000), (192, 333)]:
-        wave = zt.render(s, o, zt.Tuning(method=WAVE, precision=precision, block_size=block, queue_paths=queue))
-        assert np.array_equal(mega, wave), (name, block, queue)
+    schedules = (1, 2, 3) if precision == N.PRECISION_FP32 else (0,)
+    for block, queue in [(256, 0), (64, 64), (128, 1000), (192, 333), (256, 4097)]:
+        for sched in schedules:
+            wave = zt.render(s, o, zt.Tuning(method=WAVE, precision=precision, block_size=block, queue_paths=queue,
+                                             schedule=sched))
+            assert np.array_equal(mega, wave), (name, block, queue, sched)
 
 
 def test_wavefront_max_depth_zero_and_single_sample():
@@ -119,7 +125,8 @@ def test_rejects_bad_tuning():
     o = zt.Options(8, 8, 1, 4)
     for tu in (zt.Tuning(method=MEGA, stack_entries=7), zt.Tuning(method=MEGA, stack_entries=66),
                zt.Tuning(method=MEGA, variant=9), zt.Tuning(method=WAVE, stack_entries=5),
-               zt.Tuning(method=MEGA, bvh_width=3), zt.Tuning(method=MEGA, node_bytes=32)):
+               zt.Tuning(method=MEGA, bvh_width=3), zt.Tuning(method=MEGA, node_bytes=32),
+               zt.Tuning(method=WAVE, schedule=4), zt.Tuning(method=MEGA, schedule=3)):
         with pytest.raises(zt.ZTError) as e:
             zt.render(s, o, tu)
         assert e.value.name == "InvalidArgument"

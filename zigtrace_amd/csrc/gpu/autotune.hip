@@ -90,17 +90,24 @@ static std::vector<zt_gpu_tuning> candidates(uint32_t method, uint32_t precision
       const uint32_t v = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(q, pixels));
       if (std::find(pools.begin(), pools.end(), v) == pools.end()) pools.push_back(v);
     }
-    for (uint32_t block : {128u, 256u})
-      for (uint32_t q : pools)
-        for (uint32_t cap : {8u, 16u, 32u}) {
-          zt_gpu_tuning tu{};
-          tu.method = method;
-          tu.precision = precision;
-          tu.block_size = block;
-          tu.queue_paths = q;
-          tu.stack_entries = cap;
-          c.push_back(tu);
-        }
+    // fp32: the four-launch and the two-launch bounce of the queue-fed tracer
+    // (stack entries around its default of 12); fp64: one ray per lane.
+    const bool f32 = precision == ZT_PRECISION_FP32;
+    const std::vector<uint32_t> caps = f32 ? std::vector<uint32_t>{12, 16, 32} : std::vector<uint32_t>{8, 16, 32};
+    const std::vector<uint32_t> schedules = f32 ? std::vector<uint32_t>{2, 3} : std::vector<uint32_t>{0};
+    for (uint32_t sched : schedules)
+      for (uint32_t block : {128u, 256u})
+        for (uint32_t q : pools)
+          for (uint32_t cap : caps) {
+            zt_gpu_tuning tu{};
+            tu.method = method;
+            tu.precision = precision;
+            tu.block_size = block;
+            tu.queue_paths = q;
+            tu.stack_entries = cap;
+            tu.schedule = sched;
+            c.push_back(tu);
+          }
   }
   return c;
 }

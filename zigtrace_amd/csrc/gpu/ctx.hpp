@@ -102,20 +102,36 @@ struct DevScene {
 
 // Grow-only buffers of the wavefront renderer, reused across calls.
 struct WfBuffers {
-  DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, counters;
-  // The instantiated bounce-loop graph and the bytes of every argument it
-  // was captured with: reused while they match, captured again otherwise.
+  DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, fext, fsh, ranges, counters;
+  // The bounce-loop graph and the bytes of every argument it was captured
+  // with: reused while they match, captured again otherwise. Two
+  // instantiations of it, one per batch in flight.
   std::vector<unsigned char> graph_key;
   hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec[2] = {};
+  hipEvent_t done[2] = {};   // the batch in flight in slot i has finished
+  uint32_t* host = nullptr;  // pinned: the counter words each batch ends with, [slot][8]
   void dropGraph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
+    for (auto& e : exec) {
+      if (e) (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
     if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
     graph = nullptr;
     graph_key.clear();
   }
-  ~WfBuffers() { dropGraph(); }
+  // The events and the pinned words, created on first use.
+  void ensureHost() {
+    if (host) return;
+    for (auto& e : done) ZT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ZT_HIP(hipHostMalloc((void**)&host, 2 * 8 * sizeof(uint32_t), hipHostMallocDefault));
+  }
+  ~WfBuffers() {
+    dropGraph();
+    for (auto& e : done)
+      if (e) (void)hipEventDestroy(e);
+    if (host) (void)hipHostFree(host);
+  }
 };
 
 // Autotuner record: the fastest configuration of one method for one class of

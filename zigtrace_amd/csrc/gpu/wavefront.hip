@@ -15,8 +15,13 @@
 // megakernel (beginPath / shade / afterShadow / traverse), so the image is
 // bit-identical to it for any pool size, block size or queue order.
 //
-// Queue appends are compacted per wave: a ballot gives each lane its rank
-// and one atomic per wave reserves the wave's range.
+// Queues are kept in slot order: a kernel marks each slot that goes on to a
+// queue with a byte flag, and a three-launch stream compaction (count per
+// range of 4096 slots, scan of the counts, ordered scatter) turns the flags
+// into the queue. A slot keeps its pixel for all of that pixel's samples and
+// slots claim pixels in order, so neighbouring queue entries are neighbouring
+// pixels: a wave of the tracer walks rays from one small part of the image,
+// and the slot records a wave loads lie in a few contiguous runs.
 
 #include <algorithm>
 #include <cmath>
@@ -133,27 +138,109 @@ struct WfShared {
   double* hits_d;     // per slot: t in f64 for the fp64 path
   uint32_t* qext[2];  // extension queues (double-buffered)
   uint32_t* qsh;      // shadow queue
+  uint8_t* fext;      // per slot: 1 = on the next extension queue
+  uint8_t* fsh;       // per slot: 1 = on the shadow queue
+  uint32_t* ranges;   // compaction: flags set per range of kRangeSlots, then the range's queue offset
   // [0..1] qext counts, [2] qsh count, [4] error flag, [6..7] next pixel (u64),
   // [8..9] extension rays traced (u64), [10..11] shadow rays traced (u64),
-  // [12] / [13] read cursors of the queue-fed tracer over the extension /
-  // shadow queue (each cleared by the kernel that runs before that trace)
+  // [12] / [13] read cursors of the queue-fed tracer (and the drain) over the
+  // extension / shadow queue (each cleared by the kernel that runs before that
+  // trace)
   uint32_t* counters;
   int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
   uint32_t num_slots;
+  uint32_t num_ranges;  // slots rounded up to whole ranges of kRangeSlots
   uint64_t total_pixels;
 };
 
-// Wave-compacted append of `slot` to queue q (count at *cnt) when pred holds.
-ZT_D void append(uint32_t* q, uint32_t* cnt, uint32_t slot, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
+// Stream compaction of the slot flags, in slot order. A workgroup of
+// kRangeThreads threads covers one range of kRangeSlots flags, 16 per
+// thread (one 16-byte load).
+constexpr uint32_t kRangeThreads = 256;
+constexpr uint32_t kRangeSlots = kRangeThreads * 16;
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kMaxRanges = kScanThreads * 16;  // the scan's capacity: 2^26 slots
+
+ZT_D uint32_t flagsSet(uint4 v) {  // flags are 0 or 1: one bit per set flag
+  return (uint32_t)(__popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+ZT_D uint32_t waveInclusive(uint32_t v) {
   const uint32_t lane = __lane_id();
-  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
-  base = __shfl(base, (int)leader);
-  if (pred) q[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = slot;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d);
+    if (lane >= (uint32_t)d) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix sum over a workgroup of `kWaves` waves; *total gets the sum.
+template <int kWaves>
+ZT_D uint32_t blockExclusive(uint32_t v, uint32_t* lds, uint32_t* total) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t inc = waveInclusive(v);
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int k = 0; k < kWaves; ++k) {
+    const uint32_t t = lds[k];
+    if ((uint32_t)k < wave) before += t;
+    all += t;
+  }
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kRangeThreads) void wfCount(const uint8_t* flags, uint32_t* ranges) {
+  __shared__ uint32_t lds[kRangeThreads / 64];
+  const uint4 v = ((const uint4*)flags)[(size_t)blockIdx.x * kRangeThreads + threadIdx.x];
+  uint32_t total = 0;
+  (void)blockExclusive<kRangeThreads / 64>(flagsSet(v), lds, &total);
+  if (threadIdx.x == 0) ranges[blockIdx.x] = total;
+}
+
+// Exclusive scan of the ranges' counts in place (one workgroup); the total is
+// the queue's length.
+__global__ __launch_bounds__(kScanThreads) void wfScan(uint32_t* ranges, uint32_t n, uint32_t* count) {
+  __shared__ uint32_t lds[kScanThreads / 64];
+  const uint32_t per = (n + kScanThreads - 1) / kScanThreads;  // <= 16
+  const uint32_t first = threadIdx.x * per;
+  uint32_t c[16];
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 16; ++i) {
+    c[i] = (i < per && first + i < n) ? ranges[first + i] : 0u;
+    mine += c[i];
+  }
+  uint32_t total = 0;
+  uint32_t off = blockExclusive<kScanThreads / 64>(mine, lds, &total);
+#pragma unroll
+  for (uint32_t i = 0; i < 16; ++i) {
+    if (i < per && first + i < n) ranges[first + i] = off;
+    off += c[i];
+  }
+  if (threadIdx.x == 0) *count = total;
+}
+
+// Writes every flagged slot to q at its range's offset plus its rank in the
+// range, and clears the flags.
+__global__ __launch_bounds__(kRangeThreads) void wfScatter(uint8_t* flags, const uint32_t* ranges, uint32_t* q) {
+  __shared__ uint32_t lds[kRangeThreads / 64];
+  const size_t word = (size_t)blockIdx.x * kRangeThreads + threadIdx.x;
+  const uint4 v = ((const uint4*)flags)[word];
+  const uint32_t c = flagsSet(v);
+  uint32_t total = 0;
+  uint32_t pos = ranges[blockIdx.x] + blockExclusive<kRangeThreads / 64>(c, lds, &total);
+  if (c == 0) return;
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  const uint32_t slot0 = (uint32_t)word * 16u;
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if ((w[k >> 2] >> (8 * (k & 3))) & 1u) q[pos++] = slot0 + (uint32_t)k;
+  ((uint4*)flags)[word] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 template <typename R>
@@ -167,13 +254,29 @@ ZT_D uint32_t findScene(const WfShared<R>& w, uint64_t p) {
   return lo;
 }
 
-// Starts global pixel p in slot `sl` (its stream, accumulator and first
-// sample). Returns false only if the pixel completed at once (max_depth == 0).
+// The r-th pixel (row-major index) of a width x height image in 8x8-tile
+// order: strips of 8 rows from the top, the tiles of a strip from the left,
+// a tile's pixels row by row (tiles on the right and bottom edges narrower or
+// shorter). Slots are filled in claim order, so the 64 lanes of a wave that
+// take consecutive queue entries hold the pixels of one tile.
+ZT_D uint32_t tileOrderPixel(uint32_t r, uint32_t width, uint32_t height) {
+  const uint32_t y0 = r / (8 * width) * 8;
+  const uint32_t h = min(8u, height - y0);
+  const uint32_t i = r - y0 * width;  // within the strip
+  const uint32_t x0 = i / (8 * h) * 8;
+  const uint32_t tw = min(8u, width - x0);
+  const uint32_t j = i - x0 * h;  // within the tile
+  return (y0 + j / tw) * width + x0 + j % tw;
+}
+
+// Starts the pixel of global claim index p in slot `sl` (its stream,
+// accumulator and first sample). Returns false only if the pixel completed at
+// once (max_depth == 0).
 template <typename R>
 ZT_D bool startPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
   const uint32_t si = findScene(w, p);
   const WfScene<R>& sc = w.scenes[si];
-  const uint32_t local = (uint32_t)(p - sc.first_pixel);
+  const uint32_t local = tileOrderPixel((uint32_t)(p - sc.first_pixel), sc.pr.width, sc.pr.height);
   sl.scene = si;
   sl.pixel = local;
   sl.sample = 0;
@@ -220,30 +323,30 @@ ZT_D bool finishPath(const WfShared<R>& w, WfSlot<R>& sl) {
   return claimPixel(w, sl);
 }
 
+// Starts the first pixels in the slots and flags those with a ray for the
+// first extension queue; clears the flags of the padding to whole ranges.
 template <typename R>
 __global__ void wfInit(WfShared<R> w) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_slots; s += stride) {
-    WfSlot<R> sl{};
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_ranges * kRangeSlots; s += stride) {
     bool go = false;
-    sl.scene = 0xFFFFFFFFu;
-    if (s < w.total_pixels) go = startPixel(w, sl, s) || claimPixel(w, sl);
-    storeSlot(w.slots, s, sl);
-    append(w.qext[0], &w.counters[0], s, go);
+    if (s < w.num_slots) {
+      WfSlot<R> sl{};
+      sl.scene = 0xFFFFFFFFu;
+      if (s < w.total_pixels) go = startPixel(w, sl, s) || claimPixel(w, sl);
+      storeSlot(w.slots, s, sl);
+    }
+    w.fext[s] = go ? 1 : 0;
+    w.fsh[s] = 0;
   }
 }
 
 // Closest-hit (extension queue) or any-hit (shadow queue) query of every
-// queued ray, one ray per lane over a grid-stride loop. The first lane also
-// clears the counters that the kernels after it append to.
+// queued ray, one ray per lane over a grid-stride loop.
 template <typename R>
 __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shadow, int cap) {
   extern __shared__ int lds[];
   const LaneStack stk = laneStack(lds, cap, w.spill);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && !shadow) {
-    w.counters[1 - cur] = 0;  // next extension queue
-    w.counters[2] = 0;        // shadow queue
-  }
   const uint32_t n = shadow ? w.counters[2] : w.counters[cur];
   if (blockIdx.x == 0 && threadIdx.x == 0) *(unsigned long long*)&w.counters[shadow ? 10 : 8] += n;
   const uint32_t* q = shadow ? w.qsh : w.qext[cur];
@@ -289,8 +392,9 @@ struct WfMany {
 // passes: lanes without a ray take the next queue entries (one atomic per
 // wave on the queue's cursor), and the wave stops its passes to take more
 // once `refill` lanes have finished, until the queue is exhausted. (Cutting
-// the queue into eight chunks taken first by the waves of one XCD each was
-// measured 2x slower on the 1080p mesh field.)
+// the slot-ordered queue into eight parts, one per XCD, each XCD's waves
+// taking from their own part first, was measured 1.44x slower on the 1080p
+// mesh field: 424 vs 294 ms per frame.)
 template <bool Q, typename Src>
 ZT_D void wfTraceWaveBody(const Src& src, const WfShared<float>& w, int cur, int shadow, int cap, uint32_t refill) {
   extern __shared__ int lds[];
@@ -300,13 +404,7 @@ ZT_D void wfTraceWaveBody(const Src& src, const WfShared<float>& w, int cur, int
   const uint32_t lane = threadIdx.x;
   const bool any = shadow != 0;
   const uint32_t n = any ? w.counters[2] : w.counters[cur];
-  if (blockIdx.x == 0 && lane == 0) {
-    *(unsigned long long*)&w.counters[any ? 10 : 8] += n;
-    if (!any) {  // the queues the kernels after this one append to
-      w.counters[1 - cur] = 0;
-      w.counters[2] = 0;
-    }
-  }
+  if (blockIdx.x == 0 && lane == 0) *(unsigned long long*)&w.counters[any ? 10 : 8] += n;
   const uint32_t* q = any ? w.qsh : w.qext[cur];
   uint32_t* cursor = &w.counters[any ? 13 : 12];
   bool tracing = false, drained = false;
@@ -367,6 +465,121 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
   wfTraceWaveBody<Q>(src, w, cur, shadow, cap, refill);
 }
 
+// The two-launch bounce (fp32, four-wide trees): the queue-fed wave tracer
+// above, with each path's bounce applied by its lane in the same kernel.
+// Lanes whose query ended wait, holding the hit, until the wave stops its
+// passes to take more rays; then they load their slots together, apply the
+// bounce, store the slots and flag them for the next queues.
+//   extension launch: a finished walk is shaded (render.zig:73-99) and the
+//     slot flagged for the shadow queue if a next-event ray is pending, for
+//     the next extension queue if the path continues or its pixel starts
+//     another sample;
+//   shadow launch: a finished any-hit query is applied (render.zig:51), and
+//     a path that ended at that bounce moves on to its pixel's next sample or
+//     to a new pixel.
+template <bool Q, bool kShadowPass, typename Src>
+ZT_D void wfBounceBody(const Src& src, const WfShared<float>& w, int cur, int cap, uint32_t refill) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
+  unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
+  uint8_t* owner = (uint8_t*)(keys + 64);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = kShadowPass ? w.counters[2] : w.counters[cur];
+  if (blockIdx.x == 0 && lane == 0) {
+    *(unsigned long long*)&w.counters[kShadowPass ? 10 : 8] += n;
+    w.counters[kShadowPass ? 12 : 13] = 0;  // the cursor of the launch after this one
+  }
+  const uint32_t* q = kShadowPass ? w.qsh : w.qext[cur];
+  uint32_t* cursor = &w.counters[kShadowPass ? 13 : 12];
+  bool tracing = false, pending = false, drained = false;
+  bool ends = false;  // shadow launch: the path ends at this bounce
+  uint32_t slot = 0, si = 0;
+  HitRec<float> hit{0.0f, -1};
+  Query4 qy{};
+  Trav4 tr;
+  int overflow = 0;
+  for (;;) {
+    if (pending) {
+      pending = false;
+      if constexpr (kShadowPass) {
+        if (ends) {  // the whole slot moves on
+          WfSlot<float> sl;
+          loadSlot(w.slots, slot, sl);
+          (void)afterShadow<float>(hit, sl.p);
+          const bool to_ext = finishPath(w, sl);
+          storeSlot(w.slots, slot, sl);
+          if (to_ext) w.fext[slot] = 1;
+        } else if (hit.prim < 0) {  // only the radiance changes
+          Path<float> p;
+          p.radiance = xyz<float>(w.slots.rad[slot]);
+          p.sh_contrib = xyz<float>(w.slots.shc[slot]);
+          p.terminated = 0;
+          (void)afterShadow<float>(hit, p);
+          w.slots.rad[slot] = pack4<float>(p.radiance, 0.0f);
+        }
+      } else {
+        WfSlot<float> sl;
+        loadSlot(w.slots, slot, sl);
+        const int ph = shade<float>(src.view(si), w.scenes[Src::kSingle ? 0 : si].pr, hit, sl.rng, sl.p);
+        bool to_ext = ph == kExtend || (ph == kShadow && !sl.p.terminated);
+        if (ph == kFinish) to_ext = finishPath(w, sl);
+        storeSlot(w.slots, slot, sl);
+        if (ph == kShadow) w.fsh[slot] = 1;
+        if (to_ext) w.fext[slot] = 1;
+      }
+    }
+    if (!drained) {
+      const uint64_t need = __ballot(!tracing);
+      if (need) {
+        const uint32_t k = (uint32_t)__popcll(need);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, k);
+        base = __shfl(base, 0);
+        if (base + k >= n) drained = true;
+        const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (!tracing && idx < n) {
+          slot = q[idx];
+          si = w.slots.where[slot].x;
+          const float4 o4 = w.slots.ro[slot], d4 = kShadowPass ? w.slots.shd[slot] : w.slots.rd[slot];
+          const int4 m = w.slots.meta[slot];
+          ends = m.w != 0;
+          qy = Query4{xyz<float>(o4), xyz<float>(d4), (float)kRayEps, m.x, kShadowPass ? m.y : -1, kShadowPass};
+          tr.start(qy, kShadowPass ? d4.w : INFINITY);
+          tracing = true;
+        }
+      }
+    }
+    if (__ballot(tracing) == 0) {
+      if (drained) break;
+      continue;
+    }
+    for (;;) {
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      if (!drained && (uint32_t)__popcll(__ballot(!tracing)) >= refill) break;
+      if (wavePass<Q>(src, si, tr, qy, tracing, at_leaf, n_leaf, n_node, stk, &overflow, keys, owner)) {
+        hit = tr.finish(src.view(si), qy);
+        tracing = false;
+        pending = true;
+      }
+    }
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+template <bool Q, bool kShadowPass>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void wfBounceOne(
+    WfOne src, WfShared<float> w, int cur, int cap, uint32_t refill) {
+  wfBounceBody<Q, kShadowPass>(src, w, cur, cap, refill);
+}
+template <bool Q, bool kShadowPass>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void wfBounceMany(
+    WfMany src, WfShared<float> w, int cur, int cap, uint32_t refill) {
+  wfBounceBody<Q, kShadowPass>(src, w, cur, cap, refill);
+}
+
 // The tail of a frame: once every pixel has been claimed and the live paths
 // no longer fill the GPU, each remaining slot -- whose next extension ray is
 // pending after a full bounce iteration -- is finished by a persistent
@@ -385,7 +598,7 @@ struct DrainSrc {
 
 template <bool Q, bool kSingle>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfDrain(
-    WfShared<float> w, uint32_t n, int cap, uint32_t refill) {
+    WfShared<float> w, int cap, uint32_t refill) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
   extern __shared__ int lds[];
   const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
@@ -393,6 +606,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
   uint8_t* owner = (uint8_t*)(keys + 64);
   const DrainSrc<kSingle> src{w.scenes};
   const uint32_t* list = w.qext[0];
+  const uint32_t n = w.counters[0];  // the live extension queue after an even number of bounces
   uint32_t* cursor = &w.counters[12];
   const uint32_t lane = threadIdx.x;
   bool done = false;  // the list is used up (wave-uniform)
@@ -496,66 +710,49 @@ __global__ __launch_bounds__(256) void wfShade(WfShared<R> w, int cur) {
   const uint32_t n = w.counters[cur];
   if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[13] = 0;  // the shadow trace's cursor
   const uint32_t stride = gridDim.x * blockDim.x;
-  const int nxt = 1 - cur;
-  // Uniform trip count across the wave so every lane reaches the appends.
-  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint32_t b = base0; b < n; b += stride) {
-    const uint32_t i = b + (threadIdx.x & 63u);
-    const bool active = i < n;
-    uint32_t s = 0;
-    bool to_sh = false, to_ext = false;
-    if (active) {
-      s = w.qext[cur][i];
-      WfSlot<R> sl;
-      loadSlot(w.slots, s, sl);
-      const WfScene<R>& sc = w.scenes[sl.scene];
-      HitRec<R> h;
-      if constexpr (K<R>::exact) h.t = w.hits_d[s];
-      else h.t = w.hits[s].x;
-      h.prim = __float_as_int(w.hits[s].y);
-      const int ph = shade<R>(sc.view, sc.pr, h, sl.rng, sl.p);
-      if (ph == kShadow) to_sh = true;
-      else if (ph == kExtend) to_ext = true;
-      else to_ext = finishPath(w, sl);
-      if (ph == kShadow && !sl.p.terminated) to_ext = true;
-      storeSlot(w.slots, s, sl);
-    }
-    append(w.qsh, &w.counters[2], s, to_sh);
-    append(w.qext[nxt], &w.counters[nxt], s, to_ext);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t s = w.qext[cur][i];
+    WfSlot<R> sl;
+    loadSlot(w.slots, s, sl);
+    const WfScene<R>& sc = w.scenes[sl.scene];
+    HitRec<R> h;
+    if constexpr (K<R>::exact) h.t = w.hits_d[s];
+    else h.t = w.hits[s].x;
+    h.prim = __float_as_int(w.hits[s].y);
+    const int ph = shade<R>(sc.view, sc.pr, h, sl.rng, sl.p);
+    bool to_ext = ph == kExtend || (ph == kShadow && !sl.p.terminated);
+    if (ph == kFinish) to_ext = finishPath(w, sl);
+    storeSlot(w.slots, s, sl);
+    if (ph == kShadow) w.fsh[s] = 1;
+    if (to_ext) w.fext[s] = 1;
   }
 }
 
 // Applies each shadow ray's outcome (render.zig:51) and finishes the paths
 // that ended at that bounce.
 template <typename R>
-__global__ __launch_bounds__(256) void wfResolve(WfShared<R> w, int nxt) {
+__global__ __launch_bounds__(256) void wfResolve(WfShared<R> w) {
   const uint32_t n = w.counters[2];
   if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[12] = 0;  // the next extension trace's cursor
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint32_t b = base0; b < n; b += stride) {
-    const uint32_t i = b + (threadIdx.x & 63u);
-    uint32_t s = 0;
-    bool to_ext = false;
-    if (i < n) {
-      s = w.qsh[i];
-      const HitRec<R> h{R(0), __float_as_int(w.hits[s].y)};
-      if (w.slots.meta[s].w) {  // the path ends at this bounce: the whole slot moves on
-        WfSlot<R> sl;
-        loadSlot(w.slots, s, sl);
-        (void)afterShadow<R>(h, sl.p);
-        to_ext = finishPath(w, sl);
-        storeSlot(w.slots, s, sl);
-      } else if (h.prim < 0) {  // only the radiance changes (afterShadow, render.zig:51)
-        Path<R> p;
-        p.radiance = xyz<R>(w.slots.rad[s]);
-        p.sh_contrib = xyz<R>(w.slots.shc[s]);
-        p.terminated = 0;
-        (void)afterShadow<R>(h, p);
-        w.slots.rad[s] = pack4<R>(p.radiance, R(0));
-      }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t s = w.qsh[i];
+    const HitRec<R> h{R(0), __float_as_int(w.hits[s].y)};
+    if (w.slots.meta[s].w) {  // the path ends at this bounce: the whole slot moves on
+      WfSlot<R> sl;
+      loadSlot(w.slots, s, sl);
+      (void)afterShadow<R>(h, sl.p);
+      const bool to_ext = finishPath(w, sl);
+      storeSlot(w.slots, s, sl);
+      if (to_ext) w.fext[s] = 1;
+    } else if (h.prim < 0) {  // only the radiance changes (afterShadow, render.zig:51)
+      Path<R> p;
+      p.radiance = xyz<R>(w.slots.rad[s]);
+      p.sh_contrib = xyz<R>(w.slots.shc[s]);
+      p.terminated = 0;
+      (void)afterShadow<R>(h, p);
+      w.slots.rad[s] = pack4<R>(p.radiance, R(0));
     }
-    append(w.qext[nxt], &w.counters[nxt], s, to_ext);
   }
 }
 
@@ -589,6 +786,8 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront block size must be 64, 128, 192 or 256");
   uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(total, 1ull << 22);
   const uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
+  const uint32_t nranges = (nslots + kRangeSlots - 1) / kRangeSlots;
+  if (nranges > kMaxRanges) throw Error(ZT_ERR_INTERNAL, "wavefront slot pool exceeds the compaction's capacity");
 
   WfBuffers& b = ctx->wf;
   b.scenes.ensure(scenes.size() * sizeof(WfScene<R>));
@@ -598,7 +797,11 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   b.qext0.ensure((size_t)nslots * 4);
   b.qext1.ensure((size_t)nslots * 4);
   b.qsh.ensure((size_t)nslots * 4);
+  b.fext.ensure((size_t)nranges * kRangeSlots);
+  b.fsh.ensure((size_t)nranges * kRangeSlots);
+  b.ranges.ensure((size_t)nranges * 4);
   b.counters.ensure(16 * sizeof(uint32_t));
+  b.ensureHost();
   ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
                         ctx->stream));
 
@@ -630,14 +833,20 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.qext[0] = b.qext0.as<uint32_t>();
   w.qext[1] = b.qext1.as<uint32_t>();
   w.qsh = b.qsh.as<uint32_t>();
+  w.fext = b.fext.as<uint8_t>();
+  w.fsh = b.fsh.as<uint8_t>();
+  w.ranges = b.ranges.as<uint32_t>();
   w.counters = b.counters.as<uint32_t>();
   w.num_scenes = (uint32_t)scenes.size();
   w.num_slots = nslots;
+  w.num_ranges = nranges;
   w.total_pixels = total;
 
   // fp32 scenes that all have four-wide trees are traced by the queue-fed
-  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead).
-  bool wave_trace = false, quant = false;
+  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead);
+  // schedule 3 shades inside the tracer (two launches per bounce).
+  if (tu.schedule > 3) throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront schedule must be 0..3");
+  bool wave_trace = false, quant = false, fused = false;
   if constexpr (!K<R>::exact) {
     wave_trace = tu.schedule != 1;
     quant = scenes[0].view.nodes4q != nullptr;
@@ -646,6 +855,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
       if ((sc.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
     }
   }
+  fused = wave_trace && tu.schedule == 3;
   const int cap = tu.stack_entries ? (int)tu.stack_entries : (wave_trace ? 12 : kLds);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
   const uint32_t refill = tu.refill ? tu.refill : 32;
@@ -661,6 +871,18 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
                  : (quant ? (const void*)wfTraceWaveMany<true> : (const void*)wfTraceWaveMany<false>);
     g_wave = (int)residentBlocks(wfn, 64, wlds);
   }
+  const void* bfn[2] = {nullptr, nullptr};  // the two-launch bounce: extension, shadow
+  int g_bounce[2] = {0, 0};
+  if (fused) {
+    if (single) {
+      bfn[0] = quant ? (const void*)wfBounceOne<true, false> : (const void*)wfBounceOne<false, false>;
+      bfn[1] = quant ? (const void*)wfBounceOne<true, true> : (const void*)wfBounceOne<false, true>;
+    } else {
+      bfn[0] = quant ? (const void*)wfBounceMany<true, false> : (const void*)wfBounceMany<false, false>;
+      bfn[1] = quant ? (const void*)wfBounceMany<true, true> : (const void*)wfBounceMany<false, true>;
+    }
+    for (int k = 0; k < 2; ++k) g_bounce[k] = (int)residentBlocks(bfn[k], 64, wlds);
+  }
   // The tail hand-off (wfDrain): once every pixel is claimed and no more
   // live paths remain than the drain kernel holds lanes.
   const void* dfn = nullptr;
@@ -674,9 +896,10 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   const int g_tr = wave_trace ? 0 : gridFor<R>((const void*)wfTrace<R>, threads, lds);
   const int g_shade = gridFor<R>((const void*)wfShade<R>, threads, 0);
   const int g_res = gridFor<R>((const void*)wfResolve<R>, threads, 0);
-  const int g_init = std::max(1, (int)std::min<uint64_t>((nslots + threads - 1) / threads, 65535));
+  const int g_init = std::max(1, (int)std::min<uint64_t>(((uint64_t)nranges * kRangeSlots + threads - 1) / threads, 65535));
   ctx->spill.ensure(spillBytes(std::max<uint64_t>({(uint64_t)g_tr * threads, (uint64_t)g_wave * 64,
-                                                    (uint64_t)g_drain * 64})));
+                                                    (uint64_t)g_drain * 64, (uint64_t)g_bounce[0] * 64,
+                                                    (uint64_t)g_bounce[1] * 64})));
   w.spill = ctx->spill.as<int>();
   WfOne one;
   WfMany many;
@@ -686,21 +909,47 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     one.sc = scenes[0].view;
     many.scenes = w.scenes;
   }
-  auto trace = [&](int cur, int shadow) {
-    if (wave_trace) {
-      int c = cap;
-      uint32_t r = refill;
-      void* args[] = {single ? (void*)&one : (void*)&many, (void*)&w, &cur, &shadow, &c, &r};
-      ZT_HIP(hipLaunchKernel(wfn, dim3(g_wave), dim3(64), args, wlds, ctx->stream));
-    } else {
-      hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, ctx->stream, w, cur, shadow, cap);
+  const hipStream_t st = ctx->stream;
+  // Ordered compaction of a flag array into queue q, its length into counters[c].
+  auto compact = [&](uint8_t* flags, uint32_t* q, int c) {
+    hipLaunchKernelGGL(wfCount, dim3(nranges), dim3(kRangeThreads), 0, st, flags, w.ranges);
+    hipLaunchKernelGGL(wfScan, dim3(1), dim3(kScanThreads), 0, st, w.ranges, nranges, w.counters + c);
+    hipLaunchKernelGGL(wfScatter, dim3(nranges), dim3(kRangeThreads), 0, st, flags, w.ranges, q);
+  };
+  // One bounce: the kernels of the chosen schedule.
+  auto bounce = [&](int cur) {
+    int c = cur, cp = cap;
+    uint32_t r = refill;
+    void* src = single ? (void*)&one : (void*)&many;
+    if (fused) {
+      void* args[] = {src, (void*)&w, &c, &cp, &r};
+      ZT_HIP(hipLaunchKernel(bfn[0], dim3(g_bounce[0]), dim3(64), args, wlds, st));
+      compact(w.fsh, w.qsh, 2);
+      ZT_HIP(hipLaunchKernel(bfn[1], dim3(g_bounce[1]), dim3(64), args, wlds, st));
+      compact(w.fext, w.qext[1 - cur], 1 - cur);
+      return;
     }
+    auto trace = [&](int shadow) {
+      if (wave_trace) {
+        void* args[] = {src, (void*)&w, &c, &shadow, &cp, &r};
+        ZT_HIP(hipLaunchKernel(wfn, dim3(g_wave), dim3(64), args, wlds, st));
+      } else {
+        hipLaunchKernelGGL(wfTrace<R>, dim3(g_tr), dim3(threads), lds, st, w, cur, shadow, cap);
+      }
+    };
+    trace(0);
+    hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, st, w, cur);
+    compact(w.fsh, w.qsh, 2);
+    trace(1);
+    hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, st, w);
+    compact(w.fext, w.qext[1 - cur], 1 - cur);
   };
 
-  hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, ctx->stream, w.counters, (uint64_t)nslots);
-  hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, ctx->stream, w);
+  hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, st, w.counters, (uint64_t)nslots);
+  hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, st, w);
+  compact(w.fext, w.qext[0], 0);
   ZT_HIP(hipGetLastError());
-  ctx->stats.launches += 2;
+  ctx->stats.launches += 5;
 
   // Bounce iterations in batches of an even count, captured as a graph. The
   // context keeps the instantiated graph with the bytes of everything its
@@ -712,38 +961,30 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   {
     const size_t elem = sizeof(R);
     const uint64_t ints[] = {elem, threads, (uint64_t)g_shade, (uint64_t)g_res, (uint64_t)g_tr, (uint64_t)g_wave,
-                             lds, wlds, (uint64_t)cap, refill, (uint64_t)wave_trace, (uint64_t)single};
+                             lds, wlds, (uint64_t)cap, refill, (uint64_t)wave_trace, (uint64_t)single,
+                             (uint64_t)fused, (uint64_t)g_bounce[0], (uint64_t)g_bounce[1]};
     put(ints, sizeof ints);
     put(&w, sizeof w);
     put(&wfn, sizeof wfn);
+    put(bfn, sizeof bfn);
     if (wave_trace) put(single ? (const void*)&one : (const void*)&many, single ? sizeof one : sizeof many);
   }
   if (b.graph_key != key) {
     b.dropGraph();
-    hipGraph_t graph = nullptr;
-    ZT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < kBatch; ++j) {
-      const int cur = j & 1;
-      trace(cur, 0);
-      hipLaunchKernelGGL(wfShade<R>, dim3(g_shade), dim3(threads), 0, ctx->stream, w, cur);
-      trace(cur, 1);
-      hipLaunchKernelGGL(wfResolve<R>, dim3(g_res), dim3(threads), 0, ctx->stream, w, 1 - cur);
-    }
-    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    ZT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < kBatch; ++j) bounce(j & 1);
+    const hipError_t ce = hipStreamEndCapture(st, &b.graph);
     if (ce != hipSuccess) throw Error(ZT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-    hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      throw Error(ZT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    for (auto& ex : b.exec) {
+      const hipError_t ie = hipGraphInstantiate(&ex, b.graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        b.dropGraph();
+        throw Error(ZT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      }
     }
-    b.graph = graph;
-    b.exec = ex;
     b.graph_key = std::move(key);
   }
-  hipGraphExec_t exec = b.exec;
-  uint32_t active = 1;
-  uint64_t batches = 0;
+
   // Each batch advances every live path by kBatch bounces; a frame needs at
   // most spp * (max_depth + 1) of them per pixel pass, so the loop is bounded.
   uint64_t max_batches = 1;
@@ -752,42 +993,56 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     const uint64_t rounds = (total + nslots - 1) / nslots;
     max_batches = std::max<uint64_t>(max_batches, per_pixel * rounds / kBatch + 2);
   }
+  // Two batches stay in flight: after each one the live count and the next
+  // unclaimed pixel are copied to pinned memory behind an event, and the
+  // host, once a batch has ended, decides with the next one still queued
+  // whether to queue another, hand the live paths to the drain kernel
+  // (which reads the live count on the device), or stop (a batch that finds
+  // the queues empty does nothing).
+  uint64_t launched = 0, seen = 0;
+  bool drain = false, overrun = false;
   hipError_t le = hipSuccess;
-  bool drain = false;
-  while (active) {
-    le = hipGraphLaunch(exec, ctx->stream);
+  auto launchBatch = [&]() -> hipError_t {
+    const int i = (int)(launched & 1);
+    hipError_t e = hipGraphLaunch(b.exec[i], st);
+    if (e == hipSuccess) e = hipMemcpyAsync(b.host + i * 8, w.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(b.done[i], st);
+    if (e == hipSuccess) ++launched;
+    return e;
+  };
+  le = launchBatch();
+  if (le == hipSuccess) le = launchBatch();
+  while (le == hipSuccess) {
+    const int i = (int)(seen & 1);
+    le = hipEventSynchronize(b.done[i]);
     if (le != hipSuccess) break;
-    ++batches;
-    uint32_t c[8];  // live extension queue (after an even number of bounces) and the next unclaimed pixel
-    le = hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream);
-    if (le != hipSuccess) break;
-    le = hipStreamSynchronize(ctx->stream);
-    if (le != hipSuccess) break;
-    active = c[0];
+    ++seen;
+    const uint32_t* c = b.host + i * 8;
+    const uint32_t active = c[0];
     uint64_t next_pixel = 0;
     std::memcpy(&next_pixel, c + 6, 8);
-    if (dfn && active && next_pixel >= total && active <= drain_at) {
+    if (active == 0) break;
+    if (dfn && next_pixel >= total && active <= drain_at) {
+      int cp = cap;
+      uint32_t r = refill;
+      void* args[] = {(void*)&w, &cp, &r};
+      le = hipLaunchKernel(dfn, dim3((uint32_t)g_drain), dim3(64), args, wlds, st);
       drain = true;
       break;
     }
-    if (batches > max_batches) {
+    if (launched > max_batches) {
+      overrun = true;
       le = hipErrorUnknown;
       break;
     }
-  }
-  if (drain) {  // the last resolve cleared the cursor (counters[12]) the drain takes slots with
-    int c = cap;
-    uint32_t r = refill, n = active;
-    void* args[] = {(void*)&w, &n, &c, &r};
-    le = hipLaunchKernel(dfn, dim3((uint32_t)std::min<uint64_t>(g_drain, ((uint64_t)n + 63) / 64)), dim3(64), args,
-                         wlds, ctx->stream);
+    le = launchBatch();
   }
   if (le != hipSuccess) {
-    if (batches > max_batches) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
+    if (overrun) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
-  ctx->stats.launches += batches * kBatch * 4 + 1 + (drain ? 1 : 0);
-  hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, ctx->stream, w.counters, ctx->errflag.as<int>());
+  ctx->stats.launches += launched * kBatch * (fused ? 8 : 10) + 1 + (drain ? 1 : 0);
+  hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, st, w.counters, ctx->errflag.as<int>());
   ZT_HIP(hipGetLastError());
 }
 template void runWavefront<float>(zt_ctx*, const std::vector<WfScene<float>>&, const zt_gpu_tuning&);
