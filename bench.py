@@ -272,7 +272,9 @@ def main() -> int:
     ap.add_argument("--only", choices=["batch", "adjoint", "configs"], default=None,
                     help="run one side leg alone and print its record (for profiling its kernels)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="Adam steps of the adjoint leg")
-    ap.add_argument("--schedule", type=int, default=0, help="megakernel schedule with --no-tune (0/1 pixel, 2 wave)")
+    ap.add_argument("--schedule", type=int, default=0,
+                    help="schedule with --no-tune (megakernel: 0/1 pixel per lane, 2 wave-scheduled; wavefront: "
+                         "0/2 four-launch bounce, 1 one ray per lane, 3 two-launch bounce)")
     ap.add_argument("--refill", type=int, default=0, help="wave-scheduled refill threshold with --no-tune")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries with --no-tune")
     args = ap.parse_args()
@@ -316,8 +318,7 @@ def main() -> int:
     results = {}
     for name in chosen:
         tu = zt.Tuning(method=methods[name], stack_entries=args.stack)
-        if name == "megakernel":
-            tu.schedule, tu.refill = args.schedule, args.refill
+        tu.schedule, tu.refill = args.schedule, args.refill
         if not args.no_tune:
             t = time.time()
             tu, probe_ms = ctx.autotune(sid, o, method=methods[name])

@@ -96,6 +96,7 @@ def test_fp32_counts_agree_across_schedules_and_with_oracle(gpu_ctx, name, size,
     c0 = counts(gpu_ctx)
     for tu in (zt.Tuning(method=MK, schedule=1), zt.Tuning(method=WF, queue_paths=500),
                zt.Tuning(method=WF, queue_paths=4500, schedule=3), zt.Tuning(method=WF, schedule=1),
+               zt.Tuning(method=WF, queue_paths=1, schedule=4), zt.Tuning(method=WF, queue_paths=1, schedule=5),
                zt.Tuning(method=MK, tile_order=2), zt.Tuning(method=MK, bvh_width=2, schedule=1)):
         img = gpu_ctx.render(sid, o, tu)
         assert counts(gpu_ctx) == c0, tu

@@ -2,7 +2,8 @@
 any path-pool size (slot reuse across pixels, queues compacted over one or
 many ranges of 4096 slots), block size, schedule (one ray per lane, the
 queue-fed wave tracer with separate shading, the tracer shading each hit
-itself) and precision."""
+itself, the whole bounce in one launch over XCD segments, whole samples per
+launch) and precision."""
 import ctypes as C
 
 import numpy as np
@@ -27,7 +28,7 @@ def test_wavefront_equals_megakernel(name, w, h, spp, depth, precision):
     s = zt.parse(scene_text(name))
     o = zt.Options(w, h, spp, depth, seed=5)
     mega = zt.render(s, o, zt.Tuning(method=MEGA, precision=precision))
-    schedules = (1, 2, 3) if precision == N.PRECISION_FP32 else (0,)
+    schedules = (1, 2, 3, 4, 5) if precision == N.PRECISION_FP32 else (0, 4, 5)
     for block, queue in [(256, 0), (64, 64), (128, 1000), (192, 333), (256, 4097)]:
         for sched in schedules:
             wave = zt.render(s, o, zt.Tuning(method=WAVE, precision=precision, block_size=block, queue_paths=queue,
@@ -40,10 +41,28 @@ def test_wavefront_max_depth_zero_and_single_sample():
     for depth, spp in [(0, 3), (5, 1)]:
         o = zt.Options(17, 9, spp, depth, seed=1)
         mega = zt.render(s, o, zt.Tuning(method=MEGA))
-        wave = zt.render(s, o, zt.Tuning(method=WAVE, queue_paths=50))
-        assert np.array_equal(mega, wave)
-        if depth == 0:
-            assert not wave.any()
+        for sched in (0, 4, 5):
+            wave = zt.render(s, o, zt.Tuning(method=WAVE, queue_paths=50, schedule=sched))
+            assert np.array_equal(mega, wave), sched
+            if depth == 0:
+                assert not wave.any()
+
+
+@pytest.mark.parametrize("w,h,queue", [(256, 160, 1), (256, 160, 40960), (301, 7, 0), (3, 2, 0)])
+@pytest.mark.parametrize("schedule", [4, 5])
+def test_segmented_schedules(w, h, queue, schedule):
+    """The one-launch bounce and the sample passes keep 8 segments of whole
+    4096-slot ranges, each claiming pixels from its own band of the frame and
+    then from the other bands: a 40,960-pixel frame on 32,768 slots reuses
+    slots across bands (bands of 5,120 pixels on segments of 4,096 slots);
+    bands of a few pixels, or empty ones (6 pixels), leave most slots idle."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=3, width=w, height=h)
+    s, o = wl.scene, wl.options
+    mega = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1))
+    wave = zt.render(s, o, zt.Tuning(method=WAVE, schedule=schedule, queue_paths=queue))
+    assert np.array_equal(mega, wave)
+    assert np.isfinite(wave).all()
 
 
 def test_wavefront_no_nee_and_unbuilt_scene():
@@ -126,7 +145,7 @@ def test_rejects_bad_tuning():
     for tu in (zt.Tuning(method=MEGA, stack_entries=7), zt.Tuning(method=MEGA, stack_entries=66),
                zt.Tuning(method=MEGA, variant=9), zt.Tuning(method=WAVE, stack_entries=5),
                zt.Tuning(method=MEGA, bvh_width=3), zt.Tuning(method=MEGA, node_bytes=32),
-               zt.Tuning(method=WAVE, schedule=4), zt.Tuning(method=MEGA, schedule=3)):
+               zt.Tuning(method=WAVE, schedule=6), zt.Tuning(method=MEGA, schedule=3)):
         with pytest.raises(zt.ZTError) as e:
             zt.render(s, o, tu)
         assert e.value.name == "InvalidArgument"

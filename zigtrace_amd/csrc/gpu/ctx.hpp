@@ -101,6 +101,8 @@ struct DevScene {
 };
 
 // Grow-only buffers of the wavefront renderer, reused across calls.
+constexpr int kWfHostWords = 24;  // counter words copied to the host after each batch of bounces
+
 struct WfBuffers {
   DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, fext, fsh, ranges, counters;
   // The bounce-loop graph and the bytes of every argument it was captured
@@ -110,7 +112,7 @@ struct WfBuffers {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec[2] = {};
   hipEvent_t done[2] = {};   // the batch in flight in slot i has finished
-  uint32_t* host = nullptr;  // pinned: the counter words each batch ends with, [slot][8]
+  uint32_t* host = nullptr;  // pinned: the counter words each batch ends with, [slot][kWfHostWords]
   void dropGraph() {
     for (auto& e : exec) {
       if (e) (void)hipGraphExecDestroy(e);
@@ -124,7 +126,7 @@ struct WfBuffers {
   void ensureHost() {
     if (host) return;
     for (auto& e : done) ZT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    ZT_HIP(hipHostMalloc((void**)&host, 2 * 8 * sizeof(uint32_t), hipHostMallocDefault));
+    ZT_HIP(hipHostMalloc((void**)&host, 2 * kWfHostWords * sizeof(uint32_t), hipHostMallocDefault));
   }
   ~WfBuffers() {
     dropGraph();

@@ -92,19 +92,24 @@ ZT_D uint4 packRng(uint64_t a, uint64_t b) {
 ZT_D uint64_t lo64(uint4 v) { return (uint64_t)v.x | ((uint64_t)v.y << 32); }
 ZT_D uint64_t hi64(uint4 v) { return (uint64_t)v.z | ((uint64_t)v.w << 32); }
 
-template <typename R>
+// kPending = false leaves out the pending next-event ray (shd, shc): a slot
+// between the one-launch bounces never holds one.
+template <typename R, bool kPending = true>
 ZT_D void loadSlot(const WfSlots<R>& S, uint32_t i, WfSlot<R>& sl) {
-  const V4<R> a = S.ro[i], b = S.rd[i], c = S.shd[i], d = S.rad[i], e = S.thr[i], f = S.shc[i], g = S.sum[i];
+  const V4<R> a = S.ro[i], b = S.rd[i], d = S.rad[i], e = S.thr[i], g = S.sum[i];
   const int4 m = S.meta[i];
   const uint4 wh = S.where[i], r01 = S.rng01[i], r23 = S.rng23[i];
+  if constexpr (kPending) {
+    const V4<R> c = S.shd[i], f = S.shc[i];
+    sl.p.sh_d = xyz<R>(c);
+    sl.p.sh_tmax = c.w;
+    sl.p.sh_contrib = xyz<R>(f);
+  }
   sl.p.ro = xyz<R>(a);
   sl.p.prev_pdf = a.w;
   sl.p.rd = xyz<R>(b);
-  sl.p.sh_d = xyz<R>(c);
-  sl.p.sh_tmax = c.w;
   sl.p.radiance = xyz<R>(d);
   sl.p.throughput = xyz<R>(e);
-  sl.p.sh_contrib = xyz<R>(f);
   sl.sum = xyz<R>(g);
   sl.p.self = m.x;
   sl.p.sh_excl = m.y;
@@ -115,14 +120,16 @@ ZT_D void loadSlot(const WfSlots<R>& S, uint32_t i, WfSlot<R>& sl) {
   sl.sample = wh.z;
   sl.rng = Rng{lo64(r01), hi64(r01), lo64(r23), hi64(r23)};
 }
-template <typename R>
+template <typename R, bool kPending = true>
 ZT_D void storeSlot(const WfSlots<R>& S, uint32_t i, const WfSlot<R>& sl) {
   S.ro[i] = pack4<R>(sl.p.ro, sl.p.prev_pdf);
   S.rd[i] = pack4<R>(sl.p.rd, R(0));
-  S.shd[i] = pack4<R>(sl.p.sh_d, sl.p.sh_tmax);
+  if constexpr (kPending) {
+    S.shd[i] = pack4<R>(sl.p.sh_d, sl.p.sh_tmax);
+    S.shc[i] = pack4<R>(sl.p.sh_contrib, R(0));
+  }
   S.rad[i] = pack4<R>(sl.p.radiance, R(0));
   S.thr[i] = pack4<R>(sl.p.throughput, R(0));
-  S.shc[i] = pack4<R>(sl.p.sh_contrib, R(0));
   S.sum[i] = pack4<R>(sl.sum, R(0));
   S.meta[i] = make_int4(sl.p.self, sl.p.sh_excl, sl.p.depth, sl.p.terminated);
   S.where[i] = make_uint4(sl.scene, sl.pixel, sl.sample, 0u);
@@ -145,7 +152,9 @@ struct WfShared {
   // [8..9] extension rays traced (u64), [10..11] shadow rays traced (u64),
   // [12] / [13] read cursors of the queue-fed tracer (and the drain) over the
   // extension / shadow queue (each cleared by the kernel that runs before that
-  // trace)
+  // trace); the one-launch bounce over XCD segments: [16..23] each pixel
+  // band's claim cursor (claims taken from the band), [24..31] / [32..39] the
+  // segments' read cursors for bounces of even / odd parity
   uint32_t* counters;
   int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
@@ -153,6 +162,16 @@ struct WfShared {
   uint32_t num_ranges;  // slots rounded up to whole ranges of kRangeSlots
   uint64_t total_pixels;
 };
+
+constexpr int kWfCounterWords = 40;
+constexpr uint32_t kSegs = 8;  // XCDs: workgroup b runs on XCD b mod 8
+
+// First claim index of pixel band k of `total` (8 bands, sizes differing by
+// at most one).
+ZT_HD uint64_t pixelBand(uint32_t k, uint64_t total) {
+  const uint64_t q = total / kSegs, r = total % kSegs;
+  return k * q + (k < r ? k : r);
+}
 
 // Stream compaction of the slot flags, in slot order. A workgroup of
 // kRangeThreads threads covers one range of kRangeSlots flags, 16 per
@@ -291,6 +310,39 @@ ZT_D bool startPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
   return false;
 }
 
+// The sample passes' form of startPixel: the slot takes the pixel of claim
+// index p with its stream and an empty accumulator, its first sample not yet
+// begun.
+template <typename R>
+ZT_D void assignPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
+  const uint32_t si = findScene(w, p);
+  const WfScene<R>& sc = w.scenes[si];
+  const uint32_t local = tileOrderPixel((uint32_t)(p - sc.first_pixel), sc.pr.width, sc.pr.height);
+  sl.scene = si;
+  sl.pixel = local;
+  sl.sample = 0;
+  sl.sum = splat(R(0));
+  sl.rng = rngForStream(sc.pr.seed, local);  // render.zig:135
+}
+
+// A slot between sample passes: its pixel, the next sample's index, the
+// pixel's stream and its running sum (64 bytes).
+ZT_D void loadPixelSlot(const WfSlots<float>& S, uint32_t i, WfSlot<float>& sl) {
+  const uint4 wh = S.where[i], r01 = S.rng01[i], r23 = S.rng23[i];
+  const float4 g = S.sum[i];
+  sl.scene = wh.x;
+  sl.pixel = wh.y;
+  sl.sample = wh.z;
+  sl.rng = Rng{lo64(r01), hi64(r01), lo64(r23), hi64(r23)};
+  sl.sum = xyz<float>(g);
+}
+ZT_D void storePixelSlot(const WfSlots<float>& S, uint32_t i, const WfSlot<float>& sl) {
+  S.where[i] = make_uint4(sl.scene, sl.pixel, sl.sample, 0u);
+  S.rng01[i] = packRng(sl.rng.s0, sl.rng.s1);
+  S.rng23[i] = packRng(sl.rng.s2, sl.rng.s3);
+  S.sum[i] = pack4<float>(sl.sum, 0.0f);
+}
+
 // Claims pixels from the global counter until one has a ray to trace.
 template <typename R>
 ZT_D bool claimPixel(const WfShared<R>& w, WfSlot<R>& sl) {
@@ -304,11 +356,54 @@ ZT_D bool claimPixel(const WfShared<R>& w, WfSlot<R>& sl) {
   }
 }
 
-// A path of the slot ended: accumulate it (render.zig:141-142) and start the
-// pixel's next sample, or write the pixel (render.zig:144) and claim another.
-// Returns true if the slot has an extension ray to trace.
+// The one-launch bounce's claims: from the slot's own pixel band `home`
+// first, then from the following bands once it is used up. `used_up` has a
+// bit per band known to be fully claimed (read once per launch by
+// bandsUsedUp, and set here when a claim comes back past the band's end), so
+// a finished pixel costs no memory round trip once every band is claimed.
+// kBegin = false (sample passes) assigns the pixel without starting its first
+// sample: it always succeeds.
+template <typename R, bool kBegin = true>
+ZT_D bool claimPixelBand(const WfShared<R>& w, WfSlot<R>& sl, uint32_t home, uint32_t& used_up) {
+  for (uint32_t j = 0; j < kSegs; ++j) {
+    const uint32_t b = (home + j) % kSegs;
+    if ((used_up >> b) & 1u) continue;
+    const uint64_t lo = pixelBand(b, w.total_pixels), len = pixelBand(b + 1, w.total_pixels) - lo;
+    for (;;) {
+      const uint32_t k = atomicAdd(&w.counters[16 + b], 1u);
+      if (k >= len) {
+        used_up |= 1u << b;
+        break;
+      }
+      if constexpr (kBegin) {
+        if (startPixel(w, sl, lo + k)) return true;
+      } else {
+        assignPixel(w, sl, lo + k);
+        return true;
+      }
+    }
+  }
+  sl.scene = 0xFFFFFFFFu;
+  return false;
+}
+
+// The bands whose claim cursors had passed their end when the launch began
+// (the cursors grow only by atomics of earlier launches here, so plain loads
+// see them).
 template <typename R>
-ZT_D bool finishPath(const WfShared<R>& w, WfSlot<R>& sl) {
+ZT_D uint32_t bandsUsedUp(const WfShared<R>& w) {
+  uint32_t m = 0;
+  for (uint32_t b = 0; b < kSegs; ++b)
+    if (w.counters[16 + b] >= pixelBand(b + 1, w.total_pixels) - pixelBand(b, w.total_pixels)) m |= 1u << b;
+  return m;
+}
+
+// A path of the slot ended: accumulate it (render.zig:141-142) and start the
+// pixel's next sample, or write the pixel (render.zig:144) and claim another
+// (from the global counter, or from pixel band `home` when it is not
+// negative). Returns true if the slot has an extension ray to trace.
+template <typename R>
+ZT_D bool finishPath(const WfShared<R>& w, WfSlot<R>& sl, int home = -1, uint32_t* used_up = nullptr) {
   const WfScene<R>& sc = w.scenes[sl.scene];
   if (isFinite(sl.p.radiance)) sl.sum = add(sl.sum, sl.p.radiance);
   if (++sl.sample < sc.pr.spp) {
@@ -320,7 +415,7 @@ ZT_D bool finishPath(const WfShared<R>& w, WfSlot<R>& sl) {
   sc.out[3 * (uint64_t)sl.pixel + 0] = sl.sum.x * inv_spp;
   sc.out[3 * (uint64_t)sl.pixel + 1] = sl.sum.y * inv_spp;
   sc.out[3 * (uint64_t)sl.pixel + 2] = sl.sum.z * inv_spp;
-  return claimPixel(w, sl);
+  return home < 0 ? claimPixel(w, sl) : claimPixelBand(w, sl, (uint32_t)home, *used_up);
 }
 
 // Starts the first pixels in the slots and flags those with a ray for the
@@ -338,6 +433,53 @@ __global__ void wfInit(WfShared<R> w) {
     }
     w.fext[s] = go ? 1 : 0;
     w.fsh[s] = 0;
+  }
+}
+
+// wfInit for the one-launch bounce: slot j of segment k starts pixel j of
+// band k (the band cursors were set past those pixels by wfResetSegments);
+// slots beyond their band's length stay idle.
+template <typename R>
+__global__ void wfInitSegments(WfShared<R> w, uint32_t seg_slots) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_slots; s += stride) {
+    const uint32_t k = s / seg_slots, j = s % seg_slots;
+    const uint64_t lo = pixelBand(k, w.total_pixels), len = pixelBand(k + 1, w.total_pixels) - lo;
+    WfSlot<R> sl{};
+    sl.scene = 0xFFFFFFFFu;
+    bool go = false;
+    uint32_t used_up = 0;
+    if (j < len) go = startPixel(w, sl, lo + j) || claimPixelBand(w, sl, k, used_up);
+    storeSlot<R, false>(w.slots, s, sl);
+    w.fext[s] = go ? 1 : 0;
+  }
+}
+
+// wfInitSegments for the sample passes (fp32): the slots take their pixels
+// without starting a sample.
+__global__ void wfInitPasses(WfShared<float> w, uint32_t seg_slots) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.num_slots; s += stride) {
+    const uint32_t k = s / seg_slots, j = s % seg_slots;
+    const uint64_t lo = pixelBand(k, w.total_pixels), len = pixelBand(k + 1, w.total_pixels) - lo;
+    bool go = false;
+    if (j < len) {
+      WfSlot<float> sl{};
+      assignPixel(w, sl, lo + j);
+      storePixelSlot(w.slots, s, sl);
+      go = true;
+    }
+    w.fext[s] = go ? 1 : 0;
+  }
+}
+
+__global__ void wfResetSegments(uint32_t* counters, uint64_t total, uint32_t seg_slots) {
+  const uint32_t i = threadIdx.x;
+  if (i < kWfCounterWords) counters[i] = 0;
+  __syncthreads();
+  if (i < kSegs) {
+    const uint64_t len = pixelBand(i + 1, total) - pixelBand(i, total);
+    counters[16 + i] = (uint32_t)(len < seg_slots ? len : seg_slots);
   }
 }
 
@@ -379,12 +521,15 @@ __global__ __launch_bounds__(256) void wfTrace(WfShared<R> w, int cur, int shado
 struct WfOne {
   static constexpr bool kSingle = true;
   SceneView<float> sc;
+  Params pr;
   ZT_D const SceneView<float>& view(uint32_t) const { return sc; }
+  ZT_D const Params& params(uint32_t) const { return pr; }
 };
 struct WfMany {
   static constexpr bool kSingle = false;
   const WfScene<float>* scenes;
   ZT_D const SceneView<float>& view(uint32_t si) const { return scenes[si].view; }
+  ZT_D const Params& params(uint32_t si) const { return scenes[si].pr; }
 };
 
 // The same queries as wfTrace (fp32, four-wide trees), traced by persistent
@@ -580,6 +725,274 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   wfBounceBody<Q, kShadowPass>(src, w, cur, cap, refill);
 }
 
+// The one-launch bounce (fp32, four-wide trees): every queued path advances
+// by one bounce inside one persistent launch. A lane takes a slot, traces its
+// extension ray, shades the hit (render.zig:73-99), traces the next-event ray
+// if one is pending and applies it (render.zig:51), and stores the slot
+// flagged for the next extension queue -- with the path's next ray, its
+// pixel's next sample, or a newly claimed pixel -- or leaves it idle. All
+// three steps run under the wave-uniform passes, lanes served together once
+// `refill` of them wait. The pending next-event ray never leaves the lane, so
+// the slot traffic per bounce is one load and one store without it.
+//
+// XCD locality: the slot pool is 8 segments of `rps` compaction ranges each;
+// segment k holds the pixels of band k (its slots claim from that band
+// first), and its queue entries are contiguous in the slot-ordered queue
+// (from ranges[k * rps]). Workgroup b (on XCD b mod 8) takes entries from
+// segment b mod 8, then from the following segments once that is used up,
+// so each XCD's L2 serves the rays of one band of the image.
+template <bool Q, typename Src>
+ZT_D void wfStepBody(const Src& src, const WfShared<float>& w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
+  unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
+  uint8_t* owner = (uint8_t*)(keys + 64);
+  const uint32_t lane = threadIdx.x;
+#ifdef ZT_WF_GLOBAL_HOME
+  const uint32_t home = 0;
+#else
+  const uint32_t home = blockIdx.x % kSegs;
+#endif
+  const uint32_t n = w.counters[cur];
+  uint32_t* cursors = w.counters + 24 + 8 * cur;
+  if (blockIdx.x == 0 && lane < kSegs) w.counters[24 + 8 * (1 - cur) + lane] = 0;  // the next bounce's
+  const uint32_t* q = w.qext[cur];
+  // Wave-uniform queue position: the segment being taken from and its entries.
+  uint32_t seg_k = 0, lo = 0, hi = 0;
+  auto bounds = [&](uint32_t s) {
+    lo = w.ranges[s * rps];
+    hi = s + 1 < kSegs ? w.ranges[(s + 1) * rps] : n;
+  };
+  bool done = n == 0;
+  if (!done) bounds(home);
+  uint32_t used_up = bandsUsedUp(w);
+  bool has = false, tracing = false, sh = false;
+  uint32_t slot = 0, si = 0;
+  WfSlot<float> sl{};
+  Trav4 tr;
+  int overflow = 0;
+  uint64_t n_ext = 0, n_sh = 0;  // rays started by the wave (wave-uniform)
+  for (;;) {
+    // Serve the lanes whose query ended.
+    const bool serve = has && !tracing;
+    if (serve) {
+      const HitRec<float> hit{tr.closest, tr.best};  // what tr.finish left
+      const int ph = sh ? afterShadow<float>(hit, sl.p) : shade<float>(src.view(si), src.params(si), hit, sl.rng, sl.p);
+      if (ph == kShadow) {
+        sh = true;
+        tr.start(Query4{sl.p.ro, sl.p.sh_d, (float)kRayEps, sl.p.self, sl.p.sh_excl, true}, sl.p.sh_tmax);
+        tracing = true;
+      } else {
+        const bool to_ext = ph == kExtend || finishPath(w, sl, (int)home, &used_up);
+        if (sl.scene != 0xFFFFFFFFu) storeSlot<float, false>(w.slots, slot, sl);
+        if (to_ext) w.fext[slot] = 1;
+        has = false;
+      }
+    }
+    n_sh += (uint64_t)__popcll(__ballot(serve && tracing));
+    // Slots for the free lanes, from this XCD's segment first.
+    while (!done) {
+      const uint64_t need = __ballot(!has);
+      if (need == 0) break;
+      const uint32_t k = (uint32_t)__popcll(need);
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&cursors[(home + seg_k) % kSegs], k);
+      base = __shfl(base, 0);
+      const uint32_t idx = lo + base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+      const bool take = !has && idx < hi;
+      if (take) {
+        slot = q[idx];
+        loadSlot<float, false>(w.slots, slot, sl);
+        si = Src::kSingle ? 0u : sl.scene;
+        sh = false;
+        tr.start(Query4{sl.p.ro, sl.p.rd, (float)kRayEps, sl.p.self, -1, false}, INFINITY);
+        has = tracing = true;
+      }
+      n_ext += (uint64_t)__popcll(__ballot(take));
+      if (lo + base + k >= hi) {
+        if (++seg_k == kSegs) done = true;
+        else bounds((home + seg_k) % kSegs);
+      }
+    }
+    if (__ballot(tracing) == 0) {
+      if (done) break;
+      continue;
+    }
+    for (;;) {
+      const Query4 qy{sl.p.ro, sh ? sl.p.sh_d : sl.p.rd, (float)kRayEps, sl.p.self, sh ? sl.p.sh_excl : -1, sh};
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      if ((uint32_t)__popcll(__ballot(!tracing && (has || !done))) >= refill) break;
+      if (wavePass<Q>(src, si, tr, qy, tracing, at_leaf, n_leaf, n_node, stk, &overflow, keys, owner)) {
+        (void)tr.finish(src.view(si), qy);
+        tracing = false;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
+    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfStepOne(
+    WfOne src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  wfStepBody<Q>(src, w, cur, cap, refill, rps);
+}
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfStepMany(
+    WfMany src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  wfStepBody<Q>(src, w, cur, cap, refill, rps);
+}
+
+// Sample passes (fp32, four-wide trees): the queue holds pixels, and a take
+// runs one whole sample of the pixel -- the camera ray (render.zig:136-139),
+// every bounce with its next-event ray, the accumulation (render.zig:141-142)
+// -- under the wave-uniform passes, lanes served together once `refill` of
+// them wait; then the slot goes back on the next pass's queue with its
+// stream, sum and sample index, or the finished pixel is written
+// (render.zig:144) and the slot claims a new one for the next pass. Every
+// pixel takes exactly spp passes, so with a pool as large as the frame no
+// pixel trails the others; a pixel's samples still run in order from its own
+// stream, so the image is the megakernel's bits. Segments, bands and queue
+// cursors as in the one-launch bounce.
+template <bool Q, typename Src>
+ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  extern __shared__ int lds[];
+  const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
+  unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
+  uint8_t* owner = (uint8_t*)(keys + 64);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t home = blockIdx.x % kSegs;
+  const uint32_t n = w.counters[cur];
+  uint32_t* cursors = w.counters + 24 + 8 * cur;
+  if (blockIdx.x == 0 && lane < kSegs) w.counters[24 + 8 * (1 - cur) + lane] = 0;  // the next pass's
+  const uint32_t* q = w.qext[cur];
+  uint32_t seg_k = 0, lo = 0, hi = 0;
+  auto bounds = [&](uint32_t s) {
+    lo = w.ranges[s * rps];
+    hi = s + 1 < kSegs ? w.ranges[(s + 1) * rps] : n;
+  };
+  bool done = n == 0;
+  if (!done) bounds(home);
+  uint32_t used_up = bandsUsedUp(w);
+  bool has = false, tracing = false, sh = false;
+  uint32_t slot = 0, si = 0;
+  WfSlot<float> sl{};
+  Trav4 tr;
+  int overflow = 0;
+  uint64_t n_ext = 0, n_sh = 0;  // rays started by the wave (wave-uniform)
+  // The lane's sample ended: accumulate it and put the slot on the next
+  // pass's queue, or write the pixel and claim another.
+  auto endSample = [&]() {
+    if (isFinite(sl.p.radiance)) sl.sum = add(sl.sum, sl.p.radiance);  // render.zig:141-142
+    const Params& pr = src.params(si);
+    bool again = ++sl.sample < pr.spp;
+    if (!again) {
+      float* out = w.scenes[si].out;
+      const float inv_spp = 1.0f / (float)pr.spp;
+      out[3 * (uint64_t)sl.pixel + 0] = sl.sum.x * inv_spp;
+      out[3 * (uint64_t)sl.pixel + 1] = sl.sum.y * inv_spp;
+      out[3 * (uint64_t)sl.pixel + 2] = sl.sum.z * inv_spp;
+      again = claimPixelBand<float, false>(w, sl, home, used_up);
+    }
+    if (again) {
+      storePixelSlot(w.slots, slot, sl);
+      w.fext[slot] = 1;
+    }
+    has = false;
+  };
+  for (;;) {
+    // Serve the lanes whose query ended.
+    const bool serve = has && !tracing;
+    if (serve) {
+      const HitRec<float> hit{tr.closest, tr.best};  // what tr.finish left
+      const int ph = sh ? afterShadow<float>(hit, sl.p) : shade<float>(src.view(si), src.params(si), hit, sl.rng, sl.p);
+      if (ph == kFinish) {
+        endSample();
+      } else {
+        sh = ph == kShadow;
+        tr.start(Query4{sl.p.ro, sh ? sl.p.sh_d : sl.p.rd, (float)kRayEps, sl.p.self, sh ? sl.p.sh_excl : -1, sh},
+                 sh ? sl.p.sh_tmax : INFINITY);
+        tracing = true;
+      }
+    }
+    {
+      const uint64_t started = __ballot(serve && tracing), shadow = __ballot(serve && tracing && sh);
+      n_ext += (uint64_t)__popcll(started & ~shadow);
+      n_sh += (uint64_t)__popcll(shadow);
+    }
+    // Pixels for the free lanes, from this XCD's segment first; each starts
+    // its next sample at once.
+    while (!done) {
+      const uint64_t need = __ballot(!has);
+      if (need == 0) break;
+      const uint32_t k = (uint32_t)__popcll(need);
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&cursors[(home + seg_k) % kSegs], k);
+      base = __shfl(base, 0);
+      const uint32_t idx = lo + base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+      bool take = !has && idx < hi;
+      if (take) {
+        slot = q[idx];
+        loadPixelSlot(w.slots, slot, sl);
+        si = Src::kSingle ? 0u : sl.scene;
+        const Params& pr = src.params(si);
+        has = true;
+        if (beginPath<float>(src.view(si), pr, sl.pixel % pr.width, sl.pixel / pr.width, sl.rng, sl.p) == kExtend) {
+          sh = false;
+          tr.start(Query4{sl.p.ro, sl.p.rd, (float)kRayEps, sl.p.self, -1, false}, INFINITY);
+          tracing = true;
+        } else {  // max_depth == 0: an empty sample
+          endSample();
+          take = false;
+        }
+      }
+      n_ext += (uint64_t)__popcll(__ballot(take));
+      if (lo + base + k >= hi) {
+        if (++seg_k == kSegs) done = true;
+        else bounds((home + seg_k) % kSegs);
+      }
+    }
+    if (__ballot(tracing) == 0) {
+      if (done) break;
+      continue;
+    }
+    for (;;) {
+      const Query4 qy{sl.p.ro, sh ? sl.p.sh_d : sl.p.rd, (float)kRayEps, sl.p.self, sh ? sl.p.sh_excl : -1, sh};
+      const bool at_leaf = tracing && tr.atLeaf();
+      const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+      const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
+      if (n_leaf + n_node == 0) break;
+      if ((uint32_t)__popcll(__ballot(!tracing && (has || !done))) >= refill) break;
+      if (wavePass<Q>(src, si, tr, qy, tracing, at_leaf, n_leaf, n_node, stk, &overflow, keys, owner)) {
+        (void)tr.finish(src.view(si), qy);
+        tracing = false;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
+    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
+  }
+  if (overflow) atomicOr((int*)&w.counters[4], 1);
+}
+
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfPassOne(
+    WfOne src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  wfPassBody<Q>(src, w, cur, cap, refill, rps);
+}
+template <bool Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfPassMany(
+    WfMany src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
+  wfPassBody<Q>(src, w, cur, cap, refill, rps);
+}
+
 // The tail of a frame: once every pixel has been claimed and the live paths
 // no longer fill the GPU, each remaining slot -- whose next extension ray is
 // pending after a full bounce iteration -- is finished by a persistent
@@ -596,7 +1009,9 @@ struct DrainSrc {
   ZT_D const SceneView<float>& view(uint32_t si) const { return scene(si).view; }
 };
 
-template <bool Q, bool kSingle>
+// kPass: the slots hold pixels between sample passes (the next sample not yet
+// begun) instead of paths with a pending extension ray.
+template <bool Q, bool kSingle, bool kPass>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfDrain(
     WfShared<float> w, int cap, uint32_t refill) {
   constexpr int kResolve = 4;  // a traversal result is waiting to be applied
@@ -631,14 +1046,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         const uint32_t idx = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
         if (!has && idx < n) {
           WfSlot<float> sl;
-          loadSlot(w.slots, list[idx], sl);
+          if constexpr (kPass) loadPixelSlot(w.slots, list[idx], sl);
+          else loadSlot(w.slots, list[idx], sl);
           si = sl.scene;
           pixel = sl.pixel;
           sample = sl.sample;
           rng = sl.rng;
           sum = sl.sum;
           st = sl.p;
-          phase = kExtend;
+          phase = kPass ? (int)kCamera : (int)kExtend;
           has = true;
         }
       }
@@ -766,7 +1182,7 @@ __global__ void wfReport(const uint32_t* counters, int* status) {
 
 __global__ void wfReset(uint32_t* counters, uint64_t first_free_pixel) {
   if (threadIdx.x == 0) {
-    for (int i = 0; i < 16; ++i) counters[i] = 0;
+    for (int i = 0; i < kWfCounterWords; ++i) counters[i] = 0;
     *(unsigned long long*)&counters[6] = first_free_pixel;
   }
 }
@@ -784,8 +1200,36 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   uint32_t threads = tu.block_size ? tu.block_size : 256;
   if (threads < 64 || threads > 256 || threads % 64)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront block size must be 64, 128, 192 or 256");
+  // fp32 scenes that all have four-wide trees are traced by the queue-fed
+  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead);
+  // schedule 3 shades inside the tracer (two launches per bounce), schedule 4
+  // runs the whole bounce in one launch over XCD segments, schedule 5 runs
+  // sample passes (one launch advances every queued pixel by a whole sample).
+  if (tu.schedule > 5) throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront schedule must be 0..5");
+  bool wave_trace = false, quant = false, fused = false, step = false;
+  if constexpr (!K<R>::exact) {
+    wave_trace = tu.schedule != 1;
+    quant = scenes[0].view.nodes4q != nullptr;
+    for (const auto& sc : scenes) {
+      wave_trace = wave_trace && sc.view.nodes4 != nullptr;
+      if ((sc.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
+    }
+  }
+  fused = wave_trace && tu.schedule == 3;
+  step = wave_trace && tu.schedule == 4;
+  const bool pass = wave_trace && tu.schedule == 5;
+  const bool segmented = step || pass;  // slots in XCD segments, pixels in bands
+  if (segmented && total >= (1ull << 31)) throw Error(ZT_ERR_IMAGE_TOO_LARGE, "segmented schedules take < 2^31 pixels");
+
   uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(total, 1ull << 22);
-  const uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
+  uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
+  // Segmented schedules: 8 segments of whole compaction ranges.
+  uint32_t seg_slots = 0, rps = 0;
+  if (segmented) {
+    seg_slots = ((nslots + kSegs - 1) / kSegs + kRangeSlots - 1) / kRangeSlots * kRangeSlots;
+    rps = seg_slots / kRangeSlots;
+    nslots = kSegs * seg_slots;
+  }
   const uint32_t nranges = (nslots + kRangeSlots - 1) / kRangeSlots;
   if (nranges > kMaxRanges) throw Error(ZT_ERR_INTERNAL, "wavefront slot pool exceeds the compaction's capacity");
 
@@ -800,7 +1244,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   b.fext.ensure((size_t)nranges * kRangeSlots);
   b.fsh.ensure((size_t)nranges * kRangeSlots);
   b.ranges.ensure((size_t)nranges * 4);
-  b.counters.ensure(16 * sizeof(uint32_t));
+  b.counters.ensure(kWfCounterWords * sizeof(uint32_t));
   b.ensureHost();
   ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
                         ctx->stream));
@@ -842,20 +1286,6 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.num_ranges = nranges;
   w.total_pixels = total;
 
-  // fp32 scenes that all have four-wide trees are traced by the queue-fed
-  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead);
-  // schedule 3 shades inside the tracer (two launches per bounce).
-  if (tu.schedule > 3) throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront schedule must be 0..3");
-  bool wave_trace = false, quant = false, fused = false;
-  if constexpr (!K<R>::exact) {
-    wave_trace = tu.schedule != 1;
-    quant = scenes[0].view.nodes4q != nullptr;
-    for (const auto& sc : scenes) {
-      wave_trace = wave_trace && sc.view.nodes4 != nullptr;
-      if ((sc.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
-    }
-  }
-  fused = wave_trace && tu.schedule == 3;
   const int cap = tu.stack_entries ? (int)tu.stack_entries : (wave_trace ? 12 : kLds);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
   const uint32_t refill = tu.refill ? tu.refill : 32;
@@ -883,13 +1313,27 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     }
     for (int k = 0; k < 2; ++k) g_bounce[k] = (int)residentBlocks(bfn[k], 64, wlds);
   }
+  const void* sfn = nullptr;  // the one-launch bounce, or the sample pass
+  int g_step = 0;
+  if (step) {
+    sfn = single ? (quant ? (const void*)wfStepOne<true> : (const void*)wfStepOne<false>)
+                 : (quant ? (const void*)wfStepMany<true> : (const void*)wfStepMany<false>);
+  } else if (pass) {
+    sfn = single ? (quant ? (const void*)wfPassOne<true> : (const void*)wfPassOne<false>)
+                 : (quant ? (const void*)wfPassMany<true> : (const void*)wfPassMany<false>);
+  }
+  if (sfn) g_step = (int)residentBlocks(sfn, 64, wlds);
   // The tail hand-off (wfDrain): once every pixel is claimed and no more
   // live paths remain than the drain kernel holds lanes.
   const void* dfn = nullptr;
   int g_drain = 0;
   if (wave_trace) {
-    dfn = single ? (quant ? (const void*)wfDrain<true, true> : (const void*)wfDrain<false, true>)
-                 : (quant ? (const void*)wfDrain<true, false> : (const void*)wfDrain<false, false>);
+    if (pass)
+      dfn = single ? (quant ? (const void*)wfDrain<true, true, true> : (const void*)wfDrain<false, true, true>)
+                   : (quant ? (const void*)wfDrain<true, false, true> : (const void*)wfDrain<false, false, true>);
+    else
+      dfn = single ? (quant ? (const void*)wfDrain<true, true, false> : (const void*)wfDrain<false, true, false>)
+                   : (quant ? (const void*)wfDrain<true, false, false> : (const void*)wfDrain<false, false, false>);
     g_drain = (int)residentBlocks(dfn, 64, wlds);
   }
   const uint64_t drain_at = (uint64_t)g_drain * 64;
@@ -899,7 +1343,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   const int g_init = std::max(1, (int)std::min<uint64_t>(((uint64_t)nranges * kRangeSlots + threads - 1) / threads, 65535));
   ctx->spill.ensure(spillBytes(std::max<uint64_t>({(uint64_t)g_tr * threads, (uint64_t)g_wave * 64,
                                                     (uint64_t)g_drain * 64, (uint64_t)g_bounce[0] * 64,
-                                                    (uint64_t)g_bounce[1] * 64})));
+                                                    (uint64_t)g_bounce[1] * 64, (uint64_t)g_step * 64})));
   w.spill = ctx->spill.as<int>();
   WfOne one;
   WfMany many;
@@ -907,6 +1351,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   std::memset((void*)&many, 0, sizeof many);
   if constexpr (!K<R>::exact) {
     one.sc = scenes[0].view;
+    one.pr = scenes[0].pr;
     many.scenes = w.scenes;
   }
   const hipStream_t st = ctx->stream;
@@ -921,6 +1366,13 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     int c = cur, cp = cap;
     uint32_t r = refill;
     void* src = single ? (void*)&one : (void*)&many;
+    if (sfn) {
+      uint32_t rp = rps;
+      void* args[] = {src, (void*)&w, &c, &cp, &r, &rp};
+      ZT_HIP(hipLaunchKernel(sfn, dim3(g_step), dim3(64), args, wlds, st));
+      compact(w.fext, w.qext[1 - cur], 1 - cur);
+      return;
+    }
     if (fused) {
       void* args[] = {src, (void*)&w, &c, &cp, &r};
       ZT_HIP(hipLaunchKernel(bfn[0], dim3(g_bounce[0]), dim3(64), args, wlds, st));
@@ -945,8 +1397,16 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     compact(w.fext, w.qext[1 - cur], 1 - cur);
   };
 
-  hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, st, w.counters, (uint64_t)nslots);
-  hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, st, w);
+  if (segmented) {
+    hipLaunchKernelGGL(wfResetSegments, dim3(1), dim3(64), 0, st, w.counters, total, seg_slots);
+    if constexpr (!K<R>::exact) {
+      if (pass) hipLaunchKernelGGL(wfInitPasses, dim3(g_init), dim3(threads), 0, st, w, seg_slots);
+    }
+    if (step) hipLaunchKernelGGL(wfInitSegments<R>, dim3(g_init), dim3(threads), 0, st, w, seg_slots);
+  } else {
+    hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, st, w.counters, (uint64_t)nslots);
+    hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, st, w);
+  }
   compact(w.fext, w.qext[0], 0);
   ZT_HIP(hipGetLastError());
   ctx->stats.launches += 5;
@@ -962,11 +1422,13 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     const size_t elem = sizeof(R);
     const uint64_t ints[] = {elem, threads, (uint64_t)g_shade, (uint64_t)g_res, (uint64_t)g_tr, (uint64_t)g_wave,
                              lds, wlds, (uint64_t)cap, refill, (uint64_t)wave_trace, (uint64_t)single,
-                             (uint64_t)fused, (uint64_t)g_bounce[0], (uint64_t)g_bounce[1]};
+                             (uint64_t)fused, (uint64_t)g_bounce[0], (uint64_t)g_bounce[1], (uint64_t)step,
+                             (uint64_t)pass, (uint64_t)g_step, (uint64_t)rps};
     put(ints, sizeof ints);
     put(&w, sizeof w);
     put(&wfn, sizeof wfn);
     put(bfn, sizeof bfn);
+    put(&sfn, sizeof sfn);
     if (wave_trace) put(single ? (const void*)&one : (const void*)&many, single ? sizeof one : sizeof many);
   }
   if (b.graph_key != key) {
@@ -1005,7 +1467,9 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   auto launchBatch = [&]() -> hipError_t {
     const int i = (int)(launched & 1);
     hipError_t e = hipGraphLaunch(b.exec[i], st);
-    if (e == hipSuccess) e = hipMemcpyAsync(b.host + i * 8, w.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b.host + i * kWfHostWords, w.counters, kWfHostWords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                         st);
     if (e == hipSuccess) e = hipEventRecord(b.done[i], st);
     if (e == hipSuccess) ++launched;
     return e;
@@ -1017,12 +1481,18 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     le = hipEventSynchronize(b.done[i]);
     if (le != hipSuccess) break;
     ++seen;
-    const uint32_t* c = b.host + i * 8;
+    const uint32_t* c = b.host + i * kWfHostWords;
     const uint32_t active = c[0];
-    uint64_t next_pixel = 0;
-    std::memcpy(&next_pixel, c + 6, 8);
+    bool claimed = true;  // every pixel has been claimed
+    if (segmented) {
+      for (uint32_t k = 0; k < kSegs; ++k) claimed = claimed && c[16 + k] >= pixelBand(k + 1, total) - pixelBand(k, total);
+    } else {
+      uint64_t next_pixel = 0;
+      std::memcpy(&next_pixel, c + 6, 8);
+      claimed = next_pixel >= total;
+    }
     if (active == 0) break;
-    if (dfn && next_pixel >= total && active <= drain_at) {
+    if (dfn && claimed && active <= drain_at) {
       int cp = cap;
       uint32_t r = refill;
       void* args[] = {(void*)&w, &cp, &r};
@@ -1041,7 +1511,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     if (overrun) throw Error(ZT_ERR_INTERNAL, "wavefront did not drain");
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
-  ctx->stats.launches += launched * kBatch * (fused ? 8 : 10) + 1 + (drain ? 1 : 0);
+  ctx->stats.launches += launched * kBatch * (segmented ? 4 : fused ? 8 : 10) + 1 + (drain ? 1 : 0);
   hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, st, w.counters, ctx->errflag.as<int>());
   ZT_HIP(hipGetLastError());
 }
