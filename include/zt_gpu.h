@@ -68,8 +68,10 @@ typedef struct zt_gpu_tuning {
                              except that the wave-scheduled kernel hands tiles of 64..4096 pixels to one wave at
                              a time (0 = 8x8) */
   uint32_t tile_h;      /* megakernel: tile height in pixels */
-  uint32_t block_size;    /* wavefront: threads per workgroup (0 = 256) */
-  uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M)) */
+  uint32_t block_size;    /* wavefront: threads per workgroup of its one-ray-per-lane, shading and
+                             shadow-resolution kernels (0 = 256) */
+  uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M); schedules 4 and 5 round
+                             up to 8 segments of whole 4096-slot ranges) */
   uint32_t variant;       /* megakernel build: 1 = registers unconstrained, 2/3/4 = compiled for >= 4/5/6 waves
                              per SIMD; 0 = default (wave-scheduled: 2; one pixel per lane: fp32 4, fp64 1) */
   uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64 (0 = wave-scheduled: 12;
@@ -84,17 +86,21 @@ typedef struct zt_gpu_tuning {
                              wavefront: 1 = one ray per lane; 2 = the queue-fed tracer under the same
                              wave-uniform passes (fp32 over four-wide nodes), with shading and shadow
                              resolution in kernels of their own (four launches per bounce); 3 = that tracer
-                             shading each hit itself (two launches per bounce); 0 = 2 where it applies,
-                             else 1 */
+                             shading each hit itself (two launches per bounce); 4 = the whole bounce (trace,
+                             shade, next-event ray) in one launch, slots and pixels in 8 XCD segments and
+                             bands; 5 = sample passes: the queue holds pixels and one launch runs a whole
+                             sample of each; 0 = 5 where it applies, else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
-                             1..64 (0 = 32) */
+                             1..64 (0 = 32; wavefront sample passes: 56) */
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,
                              2 = most expensive first (a one-sample probe render without next-event rays counts
                              the extension rays of every tile, then the tiles of each band are sorted by that
                              count), 0 = 2 for a single scene with at least 16 samples per pixel and more
-                             tiles than the GPU holds waves at once, else 1 */
+                             tiles than the GPU holds waves at once, else 1.
+                             wavefront sample passes: the same order for the pixels the slots take (8x8 tiles),
+                             0 = 2 for a single scene with at least 16 samples per pixel, else 1 */
 } zt_gpu_tuning;
 
 /* Device-side timing and integer work counters of the last render call. */

@@ -12,7 +12,7 @@ from zigtrace_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("queue,schedule", [(0, 0), (4096, 0), (0, 3), (4500, 3), (700, 1), (0, 4), (1, 4), (0, 5), (1, 5)])
+@pytest.mark.parametrize("queue,schedule", [(0, 0), (4096, 2), (0, 3), (4500, 3), (700, 1), (0, 4), (1, 4), (0, 5), (1, 5)])
 def test_batch_equals_individual_renders(gpu_ctx, queue, schedule):
     ws = W.batch_scenes(count=10, seed=3, min_prims=50, max_prims=5000, resolutions=(16, 32, 48), spps=(1, 2, 4),
                         depth_range=(2, 12))

@@ -48,19 +48,22 @@ def test_wavefront_max_depth_zero_and_single_sample():
                 assert not wave.any()
 
 
-@pytest.mark.parametrize("w,h,queue", [(256, 160, 1), (256, 160, 40960), (301, 7, 0), (3, 2, 0)])
-@pytest.mark.parametrize("schedule", [4, 5])
-def test_segmented_schedules(w, h, queue, schedule):
+@pytest.mark.parametrize("w,h,queue", [(256, 160, 1), (256, 160, 40960), (301, 7, 0), (3, 2, 0), (37, 29, 1)])
+@pytest.mark.parametrize("schedule,order", [(4, 0), (5, 1), (5, 2)])
+def test_segmented_schedules(w, h, queue, schedule, order):
     """The one-launch bounce and the sample passes keep 8 segments of whole
     4096-slot ranges, each claiming pixels from its own band of the frame and
     then from the other bands: a 40,960-pixel frame on 32,768 slots reuses
     slots across bands (bands of 5,120 pixels on segments of 4,096 slots);
-    bands of a few pixels, or empty ones (6 pixels), leave most slots idle."""
+    bands of a few pixels, or empty ones (6 pixels), leave most slots idle.
+    Sample passes in longest-first order (tile_order 2) claim 64 indices per
+    8x8 tile of the probe's order, those past the right or bottom edge (301x7,
+    37x29, 3x2) being no pixel."""
     from zigtrace_amd import workloads
     wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=3, width=w, height=h)
     s, o = wl.scene, wl.options
     mega = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1))
-    wave = zt.render(s, o, zt.Tuning(method=WAVE, schedule=schedule, queue_paths=queue))
+    wave = zt.render(s, o, zt.Tuning(method=WAVE, schedule=schedule, queue_paths=queue, tile_order=order))
     assert np.array_equal(mega, wave)
     assert np.isfinite(wave).all()
 

@@ -90,24 +90,32 @@ static std::vector<zt_gpu_tuning> candidates(uint32_t method, uint32_t precision
       const uint32_t v = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(q, pixels));
       if (std::find(pools.begin(), pools.end(), v) == pools.end()) pools.push_back(v);
     }
-    // fp32: the four-launch and the two-launch bounce of the queue-fed tracer
-    // (stack entries around its default of 12); fp64: one ray per lane.
-    const bool f32 = precision == ZT_PRECISION_FP32;
-    const std::vector<uint32_t> caps = f32 ? std::vector<uint32_t>{12, 16, 32} : std::vector<uint32_t>{8, 16, 32};
-    const std::vector<uint32_t> schedules = f32 ? std::vector<uint32_t>{2, 3} : std::vector<uint32_t>{0};
-    for (uint32_t sched : schedules)
+    auto add = [&](uint32_t sched, uint32_t block, uint32_t q, uint32_t cap, uint32_t refill) {
+      zt_gpu_tuning tu{};
+      tu.method = method;
+      tu.precision = precision;
+      tu.block_size = block;
+      tu.queue_paths = q;
+      tu.stack_entries = cap;
+      tu.schedule = sched;
+      tu.refill = refill;
+      c.push_back(tu);
+    };
+    if (precision == ZT_PRECISION_FP32) {
+      // Sample passes (the two largest pools, refill and stack around their
+      // defaults), then one configuration of each bounce-level form. The
+      // block size only shapes the one-ray-per-lane kernels these do not use.
+      for (size_t i = pools.size() > 2 ? pools.size() - 2 : 0; i < pools.size(); ++i)
+        for (uint32_t refill : {48u, 56u})
+          for (uint32_t cap : {12u, 16u}) add(5, 0, pools[i], cap, refill);
+      add(4, 0, pools.back(), 16, 40);
+      for (uint32_t cap : {12u, 16u}) add(3, 0, pools.back(), cap, 0);
+      add(2, 256, pools.back(), 12, 0);
+    } else {  // one ray per lane
       for (uint32_t block : {128u, 256u})
         for (uint32_t q : pools)
-          for (uint32_t cap : caps) {
-            zt_gpu_tuning tu{};
-            tu.method = method;
-            tu.precision = precision;
-            tu.block_size = block;
-            tu.queue_paths = q;
-            tu.stack_entries = cap;
-            tu.schedule = sched;
-            c.push_back(tu);
-          }
+          for (uint32_t cap : {8u, 16u, 32u}) add(0, block, q, cap, 0);
+    }
   }
   return c;
 }

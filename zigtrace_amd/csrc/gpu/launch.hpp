@@ -28,6 +28,14 @@ struct WaveJob {
 // numbered scene after scene). Fills tiles_x / first_tile.
 void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning& tu);
 
+// Longest-first order of one scene's 8x8 tiles (fp32, four-wide BVH): the
+// wave-scheduled megakernel's one-sample cost probe without next-event rays
+// (it writes a one-sample image to `out`) and the per-band bucket sort, on
+// the context stream. Returns ctx->tile_order: each of the 8 tile bands
+// sorted most expensive first.
+const uint32_t* probeTileOrder(zt_ctx* ctx, const ztd::SceneView<float>& view, const ztd::Params& pr, float* out,
+                               int cap, uint32_t refill);
+
 // Blocks of `kernel` the device can hold at once (occupancy x CUs).
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds);
 

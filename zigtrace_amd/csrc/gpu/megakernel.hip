@@ -600,6 +600,42 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   ZT_HIP(hipGetLastError());
 }
 
+const uint32_t* probeTileOrder(zt_ctx* ctx, const SceneView<float>& view, const Params& pr, float* out, int cap,
+                               uint32_t refill) {
+  const uint32_t tw = 8, th = 8, threads = 64;
+  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
+  const uint64_t ntiles = (uint64_t)tiles_x * ((pr.height + th - 1) / th);
+  if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
+  const bool quant = view.nodes4q != nullptr;
+  const void* fn = quant ? (const void*)megakernelWave<4, true, true> : (const void*)megakernelWave<4, false, true>;
+  const size_t lds = stackBytes(cap, (int)threads) + kLeafPassBytes;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
+  ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
+  ctx->bands.ensure(8 * sizeof(uint32_t));
+  const uint32_t nt = (uint32_t)ntiles;
+  ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
+  ctx->tile_order.ensure((size_t)nt * sizeof(uint32_t));
+  uint32_t* cost = ctx->tile_cost.as<uint32_t>();
+  uint32_t* bands = ctx->bands.as<uint32_t>();
+  int* spill = ctx->spill.as<int>();
+  int* err = ctx->errflag.as<int>();
+  Params probe = pr;  // one sample, extension rays only
+  probe.spp = 1;
+  probe.use_nee = 0;
+  OneScene src{view, probe, out, tiles_x};
+  const uint32_t* no_order = nullptr;
+  void* args[] = {(void*)&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands,
+                  (void*)&no_order, (void*)&cost, &err};
+  hipLaunchKernelGGL(zeroWords, dim3((nt + 255) / 256 < 1024 ? (nt + 255) / 256 : 1024), dim3(256), 0, ctx->stream,
+                     cost, nt);
+  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
+  ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
+  hipLaunchKernelGGL(orderTiles, dim3(8), dim3(256), 0, ctx->stream, cost, ctx->tile_order.as<uint32_t>(), nt);
+  ZT_HIP(hipGetLastError());
+  ctx->stats.launches += 4;
+  return ctx->tile_order.as<uint32_t>();
+}
+
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds) {
   int dev = 0, cus = 0, per = 0;
   ZT_HIP(hipGetDevice(&dev));

@@ -160,7 +160,11 @@ struct WfShared {
   uint32_t num_scenes;
   uint32_t num_slots;
   uint32_t num_ranges;  // slots rounded up to whole ranges of kRangeSlots
-  uint64_t total_pixels;
+  uint64_t total_pixels;  // claim indices: the pixels, or 64 per 8x8 tile in `order`
+  // Sample passes over one scene in longest-first order: claim index p is
+  // pixel p % 64 (row by row) of 8x8 tile order[p / 64] (null: scan order).
+  const uint32_t* order;
+  uint32_t tiles_x;
 };
 
 constexpr int kWfCounterWords = 40;
@@ -312,17 +316,28 @@ ZT_D bool startPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
 
 // The sample passes' form of startPixel: the slot takes the pixel of claim
 // index p with its stream and an empty accumulator, its first sample not yet
-// begun.
+// begun. False if the claim index is no pixel.
 template <typename R>
-ZT_D void assignPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
-  const uint32_t si = findScene(w, p);
+ZT_D bool assignPixel(const WfShared<R>& w, WfSlot<R>& sl, uint64_t p) {
+  uint32_t si = 0, local = 0;
+  if (w.order) {  // one scene; a claim index past the image's right or bottom edge is no pixel
+    const WfScene<R>& sc = w.scenes[0];
+    const uint32_t t = w.order[p >> 6], j = (uint32_t)p & 63u;
+    const uint32_t x = (t % w.tiles_x) * 8 + (j & 7u), y = (t / w.tiles_x) * 8 + (j >> 3);
+    if (x >= sc.pr.width || y >= sc.pr.height) return false;
+    local = y * sc.pr.width + x;
+  } else {
+    si = findScene(w, p);
+    const WfScene<R>& sc = w.scenes[si];
+    local = tileOrderPixel((uint32_t)(p - sc.first_pixel), sc.pr.width, sc.pr.height);
+  }
   const WfScene<R>& sc = w.scenes[si];
-  const uint32_t local = tileOrderPixel((uint32_t)(p - sc.first_pixel), sc.pr.width, sc.pr.height);
   sl.scene = si;
   sl.pixel = local;
   sl.sample = 0;
   sl.sum = splat(R(0));
   sl.rng = rngForStream(sc.pr.seed, local);  // render.zig:135
+  return true;
 }
 
 // A slot between sample passes: its pixel, the next sample's index, the
@@ -362,7 +377,7 @@ ZT_D bool claimPixel(const WfShared<R>& w, WfSlot<R>& sl) {
 // bandsUsedUp, and set here when a claim comes back past the band's end), so
 // a finished pixel costs no memory round trip once every band is claimed.
 // kBegin = false (sample passes) assigns the pixel without starting its first
-// sample: it always succeeds.
+// sample.
 template <typename R, bool kBegin = true>
 ZT_D bool claimPixelBand(const WfShared<R>& w, WfSlot<R>& sl, uint32_t home, uint32_t& used_up) {
   for (uint32_t j = 0; j < kSegs; ++j) {
@@ -378,8 +393,7 @@ ZT_D bool claimPixelBand(const WfShared<R>& w, WfSlot<R>& sl, uint32_t home, uin
       if constexpr (kBegin) {
         if (startPixel(w, sl, lo + k)) return true;
       } else {
-        assignPixel(w, sl, lo + k);
-        return true;
+        if (assignPixel(w, sl, lo + k)) return true;
       }
     }
   }
@@ -465,9 +479,9 @@ __global__ void wfInitPasses(WfShared<float> w, uint32_t seg_slots) {
     bool go = false;
     if (j < len) {
       WfSlot<float> sl{};
-      assignPixel(w, sl, lo + j);
-      storePixelSlot(w.slots, s, sl);
-      go = true;
+      uint32_t used_up = 0;
+      go = assignPixel(w, sl, lo + j) || claimPixelBand<float, false>(w, sl, k, used_up);
+      if (go) storePixelSlot(w.slots, s, sl);
     }
     w.fext[s] = go ? 1 : 0;
   }
@@ -748,11 +762,7 @@ ZT_D void wfStepBody(const Src& src, const WfShared<float>& w, int cur, int cap,
   unsigned long long* keys = (unsigned long long*)(lds + stackBytes(cap, 64) / sizeof(int));
   uint8_t* owner = (uint8_t*)(keys + 64);
   const uint32_t lane = threadIdx.x;
-#ifdef ZT_WF_GLOBAL_HOME
-  const uint32_t home = 0;
-#else
   const uint32_t home = blockIdx.x % kSegs;
-#endif
   const uint32_t n = w.counters[cur];
   uint32_t* cursors = w.counters + 24 + 8 * cur;
   if (blockIdx.x == 0 && lane < kSegs) w.counters[24 + 8 * (1 - cur) + lane] = 0;  // the next bounce's
@@ -1200,11 +1210,13 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   uint32_t threads = tu.block_size ? tu.block_size : 256;
   if (threads < 64 || threads > 256 || threads % 64)
     throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront block size must be 64, 128, 192 or 256");
-  // fp32 scenes that all have four-wide trees are traced by the queue-fed
-  // wave-scheduled tracer (schedule 1 asks for one ray per lane instead);
-  // schedule 3 shades inside the tracer (two launches per bounce), schedule 4
-  // runs the whole bounce in one launch over XCD segments, schedule 5 runs
-  // sample passes (one launch advances every queued pixel by a whole sample).
+  // fp32 scenes that all have four-wide trees are traced under the
+  // wave-uniform passes (schedule 1 asks for one ray per lane instead):
+  // schedule 2 by the queue-fed tracer with shading and shadow resolution in
+  // kernels of their own, 3 with the tracer shading each hit itself (two
+  // launches per bounce), 4 with the whole bounce in one launch over XCD
+  // segments, 5 (the default) in sample passes (one launch advances every
+  // queued pixel by a whole sample).
   if (tu.schedule > 5) throw Error(ZT_ERR_INVALID_ARGUMENT, "wavefront schedule must be 0..5");
   bool wave_trace = false, quant = false, fused = false, step = false;
   if constexpr (!K<R>::exact) {
@@ -1215,14 +1227,28 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
       if ((sc.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
     }
   }
-  fused = wave_trace && tu.schedule == 3;
-  step = wave_trace && tu.schedule == 4;
-  const bool pass = wave_trace && tu.schedule == 5;
+  const uint32_t sched = tu.schedule ? tu.schedule : 5;  // fp32 four-wide default: sample passes
+  fused = wave_trace && sched == 3;
+  step = wave_trace && sched == 4;
+  const bool pass = wave_trace && sched == 5;
   const bool segmented = step || pass;  // slots in XCD segments, pixels in bands
-  if (segmented && total >= (1ull << 31)) throw Error(ZT_ERR_IMAGE_TOO_LARGE, "segmented schedules take < 2^31 pixels");
+  if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
+  // Sample passes over one scene take its pixels longest-first (tile_order 0:
+  // from 16 samples per pixel, so the one-sample cost probe adds at most
+  // 1/16; 1: scan order; 2: always): the slots of each band hold its most
+  // expensive 8x8 tiles first, so every pass ends on cheap pixels.
+  const bool ordered = pass && scenes.size() == 1 &&
+                       (tu.tile_order == 2 || (tu.tile_order == 0 && scenes[0].pr.spp >= 16));
+  uint32_t tiles_x = 0;
+  uint64_t claims = total;  // claim indices
+  if (ordered) {
+    tiles_x = (scenes[0].pr.width + 7) / 8;
+    claims = (uint64_t)tiles_x * ((scenes[0].pr.height + 7) / 8) * 64;
+  }
+  if (segmented && claims >= (1ull << 31)) throw Error(ZT_ERR_IMAGE_TOO_LARGE, "segmented schedules take < 2^31 pixels");
 
-  uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(total, 1ull << 22);
-  uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(total, 1ull << 26));
+  uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(claims, 1ull << 22);
+  uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(claims, 1ull << 26));
   // Segmented schedules: 8 segments of whole compaction ranges.
   uint32_t seg_slots = 0, rps = 0;
   if (segmented) {
@@ -1284,12 +1310,20 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.num_scenes = (uint32_t)scenes.size();
   w.num_slots = nslots;
   w.num_ranges = nranges;
-  w.total_pixels = total;
+  w.total_pixels = claims;
+  w.tiles_x = tiles_x;
 
   const int cap = tu.stack_entries ? (int)tu.stack_entries : (wave_trace ? 12 : kLds);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
-  const uint32_t refill = tu.refill ? tu.refill : 32;
+  // Sample passes serve later than the other forms (56 of 64 lanes waiting,
+  // against 32): a take waits on the queue cursor, the queue entry and the
+  // slot, one after the other, and serving more lanes at once spreads those
+  // waits (1080p mesh field: 48 183 ms, 56 179 ms, 64 197 ms per frame).
+  const uint32_t refill = tu.refill ? tu.refill : (pass ? 56 : 32);
   if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
+  if constexpr (!K<R>::exact) {
+    if (ordered) w.order = probeTileOrder(ctx, scenes[0].view, scenes[0].pr, scenes[0].out, cap, 32);
+  }
   const size_t lds = stackBytes(cap, (int)threads);
   const bool single = scenes.size() == 1;
   const void* wfn = nullptr;
@@ -1398,7 +1432,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   };
 
   if (segmented) {
-    hipLaunchKernelGGL(wfResetSegments, dim3(1), dim3(64), 0, st, w.counters, total, seg_slots);
+    hipLaunchKernelGGL(wfResetSegments, dim3(1), dim3(64), 0, st, w.counters, claims, seg_slots);
     if constexpr (!K<R>::exact) {
       if (pass) hipLaunchKernelGGL(wfInitPasses, dim3(g_init), dim3(threads), 0, st, w, seg_slots);
     }
@@ -1452,7 +1486,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   uint64_t max_batches = 1;
   for (const auto& s : scenes) {
     const uint64_t per_pixel = (uint64_t)s.pr.spp * ((uint64_t)std::min<uint32_t>(s.pr.max_depth, 1u << 20) + 2) * 2;
-    const uint64_t rounds = (total + nslots - 1) / nslots;
+    const uint64_t rounds = (claims + nslots - 1) / nslots;
     max_batches = std::max<uint64_t>(max_batches, per_pixel * rounds / kBatch + 2);
   }
   // Two batches stay in flight: after each one the live count and the next
@@ -1474,9 +1508,23 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     if (e == hipSuccess) ++launched;
     return e;
   };
-  le = launchBatch();
-  if (le == hipSuccess) le = launchBatch();
-  while (le == hipSuccess) {
+  // Sample passes over a pool that holds every pixel from the start (each
+  // band fits its segment): no pixel is claimed later, so the frame takes
+  // exactly max(spp) passes and the batches are queued without waiting.
+  uint32_t max_spp = 0;
+  for (const auto& s : scenes) max_spp = std::max(max_spp, s.pr.spp);
+  const bool all_at_once = pass && seg_slots >= (claims + kSegs - 1) / kSegs;
+  if (all_at_once) {
+    const uint64_t batches = (max_spp + kBatch - 1) / kBatch;
+    for (uint64_t j = 0; j < batches && le == hipSuccess; ++j) {
+      le = hipGraphLaunch(b.exec[0], st);
+      if (le == hipSuccess) ++launched;
+    }
+  } else {
+    le = launchBatch();
+    if (le == hipSuccess) le = launchBatch();
+  }
+  while (le == hipSuccess && !all_at_once) {
     const int i = (int)(seen & 1);
     le = hipEventSynchronize(b.done[i]);
     if (le != hipSuccess) break;
@@ -1485,7 +1533,8 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     const uint32_t active = c[0];
     bool claimed = true;  // every pixel has been claimed
     if (segmented) {
-      for (uint32_t k = 0; k < kSegs; ++k) claimed = claimed && c[16 + k] >= pixelBand(k + 1, total) - pixelBand(k, total);
+      for (uint32_t k = 0; k < kSegs; ++k)
+        claimed = claimed && c[16 + k] >= pixelBand(k + 1, claims) - pixelBand(k, claims);
     } else {
       uint64_t next_pixel = 0;
       std::memcpy(&next_pixel, c + 6, 8);
