@@ -197,6 +197,7 @@ def bench_batch(zt, N, W, steps: int):
             "persistent_wavefront_mpaths_s": round(paths / wavefront_s / 1e6, 2),
             "per_scene_megakernel_mpaths_s": round(paths / per_scene_s / 1e6, 2),
             "persistent_over_per_scene": round(per_scene_s / persistent_s, 3),
+            "persistent_wavefront_over_per_scene": round(per_scene_s / wavefront_s, 3),
             "prims_total": int(sum(w.scene.info()["num_shapes"] for w in wls))}
 
 

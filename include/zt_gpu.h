@@ -70,8 +70,8 @@ typedef struct zt_gpu_tuning {
   uint32_t tile_h;      /* megakernel: tile height in pixels */
   uint32_t block_size;    /* wavefront: threads per workgroup of its one-ray-per-lane, shading and
                              shadow-resolution kernels (0 = 256) */
-  uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M); schedules 4 and 5 round
-                             up to 8 segments of whole 4096-slot ranges) */
+  uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M); sample passes: every pixel,
+                             up to 64M; schedules 4 and 5 round up to 8 segments of whole 4096-slot ranges) */
   uint32_t variant;       /* megakernel build: 1 = registers unconstrained, 2/3/4 = compiled for >= 4/5/6 waves
                              per SIMD; 0 = default (wave-scheduled: 2; one pixel per lane: fp32 4, fp64 1) */
   uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64 (0 = wave-scheduled: 12;
@@ -91,7 +91,7 @@ typedef struct zt_gpu_tuning {
                              bands; 5 = sample passes: the queue holds pixels and one launch runs a whole
                              sample of each; 0 = 5 where it applies, else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
-                             1..64 (0 = 32; wavefront sample passes: 56) */
+                             1..64 (0 = 32; wavefront sample passes: 56 for one scene, 40 for a batch) */
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,

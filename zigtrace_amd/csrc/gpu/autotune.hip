@@ -102,12 +102,12 @@ static std::vector<zt_gpu_tuning> candidates(uint32_t method, uint32_t precision
       c.push_back(tu);
     };
     if (precision == ZT_PRECISION_FP32) {
-      // Sample passes (the two largest pools, refill and stack around their
-      // defaults), then one configuration of each bounce-level form. The
-      // block size only shapes the one-ray-per-lane kernels these do not use.
-      for (size_t i = pools.size() > 2 ? pools.size() - 2 : 0; i < pools.size(); ++i)
-        for (uint32_t refill : {48u, 56u})
-          for (uint32_t cap : {12u, 16u}) add(5, 0, pools[i], cap, refill);
+      // Sample passes (their default pool, which holds every pixel; refill
+      // and stack around their defaults), then one configuration of each
+      // bounce-level form. The block size only shapes the one-ray-per-lane
+      // kernels these do not use.
+      for (uint32_t refill : {48u, 56u})
+        for (uint32_t cap : {12u, 16u}) add(5, 0, 0, cap, refill);
       add(4, 0, pools.back(), 16, 40);
       for (uint32_t cap : {12u, 16u}) add(3, 0, pools.back(), cap, 0);
       add(2, 256, pools.back(), 12, 0);

@@ -1247,7 +1247,13 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   }
   if (segmented && claims >= (1ull << 31)) throw Error(ZT_ERR_IMAGE_TOO_LARGE, "segmented schedules take < 2^31 pixels");
 
-  uint64_t want = tu.queue_paths ? tu.queue_paths : std::min<uint64_t>(claims, 1ull << 22);
+  // Default pool: sample passes hold every pixel from the start (64 B each:
+  // 2^26 pixels take 4 GiB), so no slot ever takes a second pixel and the
+  // queue stays in claim order -- neighbouring entries are neighbouring
+  // pixels of one scene (the 256-scene batch: a 4M-slot pool whose slots
+  // take new pixels as theirs finish mixes scenes within waves, 487 against
+  // 618 Mpaths/s); the bounce forms hold up to 4M paths (176 B each).
+  uint64_t want = tu.queue_paths ? tu.queue_paths : pass ? claims : std::min<uint64_t>(claims, 1ull << 22);
   uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), std::min<uint64_t>(claims, 1ull << 26));
   // Segmented schedules: 8 segments of whole compaction ranges.
   uint32_t seg_slots = 0, rps = 0;
@@ -1261,14 +1267,17 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
 
   WfBuffers& b = ctx->wf;
   b.scenes.ensure(scenes.size() * sizeof(WfScene<R>));
-  b.slots.ensure((size_t)nslots * WfSlots<R>::kBytes + 11 * 256);
-  b.hits.ensure((size_t)nslots * sizeof(float2));
-  b.hits_d.ensure((size_t)nslots * sizeof(double));
+  // Sample passes keep only a slot's pixel, stream and sum (4 arrays of 16
+  // bytes) and use neither hit records nor the shadow queue.
+  const size_t path_slots = pass ? 1 : nslots;
+  b.slots.ensure((size_t)nslots * (pass ? 4 * sizeof(uint4) : WfSlots<R>::kBytes) + 11 * 256);
+  b.hits.ensure(path_slots * sizeof(float2));
+  b.hits_d.ensure(path_slots * sizeof(double));
   b.qext0.ensure((size_t)nslots * 4);
   b.qext1.ensure((size_t)nslots * 4);
-  b.qsh.ensure((size_t)nslots * 4);
+  b.qsh.ensure(path_slots * 4);
   b.fext.ensure((size_t)nranges * kRangeSlots);
-  b.fsh.ensure((size_t)nranges * kRangeSlots);
+  b.fsh.ensure(pass ? 4096 : (size_t)nranges * kRangeSlots);
   b.ranges.ensure((size_t)nranges * 4);
   b.counters.ensure(kWfCounterWords * sizeof(uint32_t));
   b.ensureHost();
@@ -1286,14 +1295,16 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
       off += ((size_t)nslots * elem + 255) & ~(size_t)255;
       return p;
     };
-    w.slots.ro = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.rd = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.shd = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.rad = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.thr = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.shc = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.sum = (V4<R>*)carve(sizeof(V4<R>));
-    w.slots.meta = (int4*)carve(sizeof(int4));
+    if (!pass) {
+      w.slots.ro = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.rd = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.shd = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.rad = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.thr = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.shc = (V4<R>*)carve(sizeof(V4<R>));
+      w.slots.meta = (int4*)carve(sizeof(int4));
+    }
+    w.slots.sum = (V4<R>*)carve(pass ? sizeof(uint4) : sizeof(V4<R>));
     w.slots.where = (uint4*)carve(sizeof(uint4));
     w.slots.rng01 = (uint4*)carve(sizeof(uint4));
     w.slots.rng23 = (uint4*)carve(sizeof(uint4));
@@ -1315,11 +1326,14 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
 
   const int cap = tu.stack_entries ? (int)tu.stack_entries : (wave_trace ? 12 : kLds);
   if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
-  // Sample passes serve later than the other forms (56 of 64 lanes waiting,
-  // against 32): a take waits on the queue cursor, the queue entry and the
-  // slot, one after the other, and serving more lanes at once spreads those
-  // waits (1080p mesh field: 48 183 ms, 56 179 ms, 64 197 ms per frame).
-  const uint32_t refill = tu.refill ? tu.refill : (pass ? 56 : 32);
+  // Sample passes serve later than the other forms: a take waits on the queue
+  // cursor, the queue entry and the slot, one after the other, and serving
+  // more lanes at once spreads those waits. One scene: 56 of 64 lanes waiting
+  // (1080p mesh field, scan order: 32 215 ms, 48 183 ms, 56 179 ms, 64 197
+  // ms per frame); many scenes, whose shorter paths make the waiting lanes'
+  // idle time dearer: 40 (the 256-scene batch: 36-44 675-678 Mpaths/s, 48
+  // 662, 56 617).
+  const uint32_t refill = tu.refill ? tu.refill : pass ? (scenes.size() == 1 ? 56 : 40) : 32;
   if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
   if constexpr (!K<R>::exact) {
     if (ordered) w.order = probeTileOrder(ctx, scenes[0].view, scenes[0].pr, scenes[0].out, cap, 32);
