@@ -27,6 +27,25 @@ def test_batch_equals_individual_renders(gpu_ctx, queue, schedule):
         gpu_ctx.release(sid)
 
 
+@pytest.mark.parametrize("schedule", [4, 5])
+def test_batch_pool_smaller_than_the_pixels(gpu_ctx, schedule):
+    """Segmented schedules over a batch with more pixels (about 88k) than the
+    smallest pool (8 segments of 4096 slots): slots take pixels of other
+    scenes as theirs finish, bands run out at different times, and the tail
+    goes to the drain (which for sample passes starts each slot's next
+    sample)."""
+    ws = W.batch_scenes(count=5, seed=9, min_prims=50, max_prims=3000, resolutions=(112, 160), spps=(1, 3, 5),
+                        depth_range=(2, 10))
+    ids = [gpu_ctx.upload(w.scene) for w in ws]
+    opts = [w.options for w in ws]
+    assert sum(o.width * o.height for o in opts) > 8 * 4096
+    outs = gpu_ctx.render_batch(ids, opts, zt.Tuning(method=N.METHOD_WAVEFRONT, queue_paths=1, schedule=schedule))
+    for sid, o, img in zip(ids, opts, outs):
+        assert np.array_equal(gpu_ctx.render(sid, o, zt.Tuning(method=N.METHOD_MEGAKERNEL)), img)
+    for sid in ids:
+        gpu_ctx.release(sid)
+
+
 def test_batch_fp64_and_repeated_scene(gpu_ctx):
     ws = W.batch_scenes(count=3, seed=5, min_prims=20, max_prims=500, resolutions=(16,), spps=(2,),
                         depth_range=(4, 8))

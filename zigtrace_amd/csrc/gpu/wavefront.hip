@@ -1335,9 +1335,6 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   // 662, 56 617).
   const uint32_t refill = tu.refill ? tu.refill : pass ? (scenes.size() == 1 ? 56 : 40) : 32;
   if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
-  if constexpr (!K<R>::exact) {
-    if (ordered) w.order = probeTileOrder(ctx, scenes[0].view, scenes[0].pr, scenes[0].out, cap, 32);
-  }
   const size_t lds = stackBytes(cap, (int)threads);
   const bool single = scenes.size() == 1;
   const void* wfn = nullptr;
@@ -1393,6 +1390,11 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
                                                     (uint64_t)g_drain * 64, (uint64_t)g_bounce[0] * 64,
                                                     (uint64_t)g_bounce[1] * 64, (uint64_t)g_step * 64})));
   w.spill = ctx->spill.as<int>();
+  // The cost probe after the spill buffer has its final size (it runs on the
+  // same buffer).
+  if constexpr (!K<R>::exact) {
+    if (ordered) w.order = probeTileOrder(ctx, scenes[0].view, scenes[0].pr, scenes[0].out, cap, 32);
+  }
   WfOne one;
   WfMany many;
   std::memset((void*)&one, 0, sizeof one);  // their bytes key the cached graph too
