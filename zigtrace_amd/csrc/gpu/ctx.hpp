@@ -104,7 +104,7 @@ struct DevScene {
 constexpr int kWfHostWords = 24;  // counter words copied to the host after each batch of bounces
 
 struct WfBuffers {
-  DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, fext, fsh, ranges, counters;
+  DevBuf scenes, slots, hits, hits_d, qext0, qext1, qsh, fext, fsh, ranges, counters, tally;
   // The bounce-loop graph and the bytes of every argument it was captured
   // with: reused while they match, captured again otherwise. Two
   // instantiations of it, one per batch in flight.

@@ -156,6 +156,10 @@ struct WfShared {
   // band's claim cursor (claims taken from the band), [24..31] / [32..39] the
   // segments' read cursors for bounces of even / odd parity
   uint32_t* counters;
+  // Ray counts of the persistent kernels that tally per wave: kTallyLines
+  // lines of 16 words (128 bytes), the extension rays in word 0, the shadow
+  // rays in word 1; wfReport sums them.
+  unsigned long long* tally;
   int* spill;  // traversal stack overflow of the trace kernel's threads
   uint32_t num_scenes;
   uint32_t num_slots;
@@ -168,6 +172,19 @@ struct WfShared {
 };
 
 constexpr int kWfCounterWords = 40;
+constexpr uint32_t kTallyLines = 64;
+static_assert(kWfCounterWords <= (int)kTallyLines, "the reset kernels clear one counter word per thread");
+
+// A wave's ray counts at exit: workgroup b adds to tally line b mod
+// kTallyLines, so the thousands of waves of a launch do not all add into the
+// same two words (8192 device-scope atomics on one address took ~0.1 ms per
+// launch, the whole cost of an empty sample pass).
+template <typename R>
+ZT_D void tallyRays(const WfShared<R>& w, uint64_t n_ext, uint64_t n_sh) {
+  unsigned long long* t = w.tally + (size_t)(blockIdx.x % kTallyLines) * 16;
+  if (n_ext) atomicAdd(t, (unsigned long long)n_ext);
+  if (n_sh) atomicAdd(t + 1, (unsigned long long)n_sh);
+}
 constexpr uint32_t kSegs = 8;  // XCDs: workgroup b runs on XCD b mod 8
 
 // First claim index of pixel band k of `total` (8 bands, sizes differing by
@@ -487,8 +504,11 @@ __global__ void wfInitPasses(WfShared<float> w, uint32_t seg_slots) {
   }
 }
 
-__global__ void wfResetSegments(uint32_t* counters, uint64_t total, uint32_t seg_slots) {
+// One workgroup of kTallyLines (64) threads.
+__global__ void wfResetSegments(uint32_t* counters, unsigned long long* tally, uint64_t total, uint32_t seg_slots) {
   const uint32_t i = threadIdx.x;
+  tally[i * 16] = 0;
+  tally[i * 16 + 1] = 0;
   if (i < kWfCounterWords) counters[i] = 0;
   __syncthreads();
   if (i < kSegs) {
@@ -841,10 +861,7 @@ ZT_D void wfStepBody(const Src& src, const WfShared<float>& w, int cur, int cap,
       }
     }
   }
-  if (lane == 0) {
-    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
-    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
-  }
+  if (lane == 0) tallyRays(w, n_ext, n_sh);
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
@@ -985,10 +1002,7 @@ ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap,
       }
     }
   }
-  if (lane == 0) {
-    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
-    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
-  }
+  if (lane == 0) tallyRays(w, n_ext, n_sh);
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
@@ -1124,10 +1138,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
       }
     }
   }
-  if (lane == 0) {
-    atomicAdd((unsigned long long*)&w.counters[8], (unsigned long long)n_ext);
-    atomicAdd((unsigned long long*)&w.counters[10], (unsigned long long)n_sh);
-  }
+  if (lane == 0) tallyRays(w, n_ext, n_sh);
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
@@ -1182,17 +1193,32 @@ __global__ __launch_bounds__(256) void wfResolve(WfShared<R> w) {
   }
 }
 
-// Adds the loop's ray counts and error flag to the launch status words.
-__global__ void wfReport(const uint32_t* counters, int* status) {
-  if (threadIdx.x == 0) {
-    addRayCounts(status, *(const unsigned long long*)&counters[8], *(const unsigned long long*)&counters[10]);
+// Adds the loop's ray counts (the counter words and the tally lines) and
+// error flag to the launch status words. One workgroup of kTallyLines threads.
+__global__ void wfReport(const uint32_t* counters, const unsigned long long* tally, int* status) {
+  __shared__ unsigned long long sum[2][kTallyLines];
+  const uint32_t i = threadIdx.x;
+  sum[0][i] = tally[i * 16];
+  sum[1][i] = tally[i * 16 + 1];
+  __syncthreads();
+  if (i == 0) {
+    unsigned long long e = *(const unsigned long long*)&counters[8], sh = *(const unsigned long long*)&counters[10];
+    for (uint32_t k = 0; k < kTallyLines; ++k) {
+      e += sum[0][k];
+      sh += sum[1][k];
+    }
+    addRayCounts(status, e, sh);
     if (counters[4]) atomicOr(status, 1);
   }
 }
 
-__global__ void wfReset(uint32_t* counters, uint64_t first_free_pixel) {
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < kWfCounterWords; ++i) counters[i] = 0;
+// One workgroup of kTallyLines threads.
+__global__ void wfReset(uint32_t* counters, unsigned long long* tally, uint64_t first_free_pixel) {
+  const uint32_t i = threadIdx.x;
+  tally[i * 16] = 0;
+  tally[i * 16 + 1] = 0;
+  if (i == 0) {
+    for (int k = 0; k < kWfCounterWords; ++k) counters[k] = 0;
     *(unsigned long long*)&counters[6] = first_free_pixel;
   }
 }
@@ -1280,6 +1306,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   b.fsh.ensure(pass ? 4096 : (size_t)nranges * kRangeSlots);
   b.ranges.ensure((size_t)nranges * 4);
   b.counters.ensure(kWfCounterWords * sizeof(uint32_t));
+  b.tally.ensure(kTallyLines * 128);
   b.ensureHost();
   ZT_HIP(hipMemcpyAsync(b.scenes.p, scenes.data(), scenes.size() * sizeof(WfScene<R>), hipMemcpyHostToDevice,
                         ctx->stream));
@@ -1318,6 +1345,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   w.fsh = b.fsh.as<uint8_t>();
   w.ranges = b.ranges.as<uint32_t>();
   w.counters = b.counters.as<uint32_t>();
+  w.tally = b.tally.as<unsigned long long>();
   w.num_scenes = (uint32_t)scenes.size();
   w.num_slots = nslots;
   w.num_ranges = nranges;
@@ -1448,13 +1476,13 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
   };
 
   if (segmented) {
-    hipLaunchKernelGGL(wfResetSegments, dim3(1), dim3(64), 0, st, w.counters, claims, seg_slots);
+    hipLaunchKernelGGL(wfResetSegments, dim3(1), dim3(kTallyLines), 0, st, w.counters, w.tally, claims, seg_slots);
     if constexpr (!K<R>::exact) {
       if (pass) hipLaunchKernelGGL(wfInitPasses, dim3(g_init), dim3(threads), 0, st, w, seg_slots);
     }
     if (step) hipLaunchKernelGGL(wfInitSegments<R>, dim3(g_init), dim3(threads), 0, st, w, seg_slots);
   } else {
-    hipLaunchKernelGGL(wfReset, dim3(1), dim3(64), 0, st, w.counters, (uint64_t)nslots);
+    hipLaunchKernelGGL(wfReset, dim3(1), dim3(kTallyLines), 0, st, w.counters, w.tally, (uint64_t)nslots);
     hipLaunchKernelGGL(wfInit<R>, dim3(g_init), dim3(threads), 0, st, w);
   }
   compact(w.fext, w.qext[0], 0);
@@ -1577,7 +1605,7 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     throw Error(ZT_ERR_HIP, std::string("wavefront loop: ") + hipGetErrorString(le));
   }
   ctx->stats.launches += launched * kBatch * (segmented ? 4 : fused ? 8 : 10) + 1 + (drain ? 1 : 0);
-  hipLaunchKernelGGL(wfReport, dim3(1), dim3(64), 0, st, w.counters, ctx->errflag.as<int>());
+  hipLaunchKernelGGL(wfReport, dim3(1), dim3(kTallyLines), 0, st, w.counters, w.tally, ctx->errflag.as<int>());
   ZT_HIP(hipGetLastError());
 }
 template void runWavefront<float>(zt_ctx*, const std::vector<WfScene<float>>&, const zt_gpu_tuning&);
