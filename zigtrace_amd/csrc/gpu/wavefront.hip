@@ -1,27 +1,33 @@
-// Wavefront renderer: the bounce loop split into three persistent kernels
-// over compacted queues of path slots, captured in a hipGraph.
+// Wavefront renderer: a pool of slots, compacted queues of them, and
+// persistent kernels that work through a queue, the loop over queues
+// captured in a hipGraph. Forms (zt_gpu_tuning.schedule):
 //
-//   extend  : closest-hit query of every queued extension ray
-//   shade   : one bounce per queued path (render.zig:73-99); appends the path
-//             to the shadow queue if a next-event ray is pending, to the next
-//             extension queue if it continues, or finishes it
-//   shadow  : any-hit query of every queued next-event ray (render.zig:51)
+//   5 sample passes (the default for fp32 four-wide trees): the queue holds
+//     pixels; one launch runs one whole sample of every queued pixel under
+//     the megakernel's wave-uniform traversal passes, lanes refilling from
+//     the queue as their paths end (wfPassBody)
+//   4 one bounce per launch: trace, shade, next-event ray (wfStepBody)
+//   3 two launches per bounce: extension, then shadow queue, each hit shaded
+//     by the lane that traced it (wfBounceBody)
+//   2 four launches per bounce: extension trace, shade, shadow trace, resolve
+//     (wfTraceWaveBody, wfShade, wfResolve)
+//   1 (and fp64) the same with one ray per lane (wfTrace)
 //
-// A bounded pool of path slots works through all pixels of all scenes in the
-// batch: a slot owns one pixel at a time (its xoshiro256++ stream and
-// accumulator), runs that pixel's samples one after the other as the
-// reference does (render.zig:133-144), then claims the next unstarted pixel
-// from a global counter. Every path runs the same device functions as the
-// megakernel (beginPath / shade / afterShadow / traverse), so the image is
-// bit-identical to it for any pool size, block size or queue order.
+// A slot owns one pixel at a time (its xoshiro256++ stream and accumulator),
+// runs that pixel's samples one after the other as the reference does
+// (render.zig:133-144), then claims the next unstarted pixel. Every path runs
+// the same device functions as the megakernel (beginPath / shade /
+// afterShadow / traverse), so the image is bit-identical to it for any form,
+// pool size, block size or queue order.
 //
 // Queues are kept in slot order: a kernel marks each slot that goes on to a
 // queue with a byte flag, and a three-launch stream compaction (count per
 // range of 4096 slots, scan of the counts, ordered scatter) turns the flags
-// into the queue. A slot keeps its pixel for all of that pixel's samples and
-// slots claim pixels in order, so neighbouring queue entries are neighbouring
-// pixels: a wave of the tracer walks rays from one small part of the image,
-// and the slot records a wave loads lie in a few contiguous runs.
+// into the queue. Slots claim pixels in order, so neighbouring queue entries
+// are neighbouring pixels: a wave walks rays from one small part of the
+// image, and the slot records it loads lie in a few contiguous runs. The
+// segmented forms (4, 5) cut the slots into 8 segments, one per XCD, whose
+// slots claim pixels from their own band of the frame.
 
 #include <algorithm>
 #include <cmath>
