@@ -64,6 +64,25 @@ def test_mesh_field_seed_means(gpu_ctx, mesh_field):
     assert_seed_means(gpu, cpu, 8, "mesh field")
 
 
+def test_mesh_field_full_size_wavefront_equals_megakernel(gpu_ctx, mesh_field):
+    """The benchmark frame (1920x1080, 1.3M triangles, depth 16) at 16 spp
+    through the wavefront's default form -- the one-sample cost probe and the
+    longest-first pixel order, a pool of every pixel, 16 sample passes -- and
+    through the default wave-scheduled megakernel: the same bits, and the
+    same extension and shadow ray counts."""
+    wl, _ = mesh_field
+    o = wl.options
+    o16 = zt.Options(o.width, o.height, 16, o.max_depth, seed=11)
+    sid = gpu_ctx.upload(wl.scene)
+    mega = gpu_ctx.render(sid, o16, zt.Tuning(method=N.METHOD_MEGAKERNEL))
+    mega_rays = {k: gpu_ctx.stats()[k] for k in ("extend_rays", "shadow_rays")}
+    wave = gpu_ctx.render(sid, o16, zt.Tuning(method=N.METHOD_WAVEFRONT))
+    wave_rays = {k: gpu_ctx.stats()[k] for k in ("extend_rays", "shadow_rays")}
+    gpu_ctx.release(sid)
+    assert np.array_equal(mega, wave)
+    assert mega_rays == wave_rays and mega_rays["extend_rays"] > o.width * o.height * 16
+
+
 def test_mesh_field_full_width_crn_rmse(gpu_ctx, mesh_field):
     wl, ref = mesh_field
     o = wl.options  # 1920x1080, depth 16, NEE + MIS
