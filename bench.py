@@ -386,6 +386,8 @@ def main() -> int:
                     "kernel": "megakernel", "kernel_ms": round(km, 3), "bytes_per_path": round(bpp, 2),
                     "bytes_per_launch": round(paths * bpp), "bytes_source": source,
                     "traffic_source": "profiles/megakernel_mesh_field_1080p64_pmc.txt" if tpp is not None else None}
+        for r_m in results.values():  # each method's frame against the same algorithmic bytes
+            r_m["roofline_frac"] = round(paths * bpp / (r_m["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     extra = {}
     if not args.no_extra:
         ctx.free(dout)
