@@ -36,6 +36,24 @@ def test_wavefront_equals_megakernel(name, w, h, spp, depth, precision):
             assert np.array_equal(mega, wave), (name, block, queue, sched)
 
 
+def test_sample_passes_beyond_one_graph():
+    """Sample passes run in replays of a 16-pass graph: 37 samples per pixel
+    take three replays whose last has 11 passes with empty queues, and a pool
+    smaller than the frame (queue_paths) claims pixels across replays and
+    ends in the drain. Both give the megakernel's bits."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=37, width=48, height=40)
+    s, o = wl.scene, wl.options
+    mega = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1))
+    for queue, order in [(0, 0), (0, 1), (1, 1)]:
+        wave = zt.render(s, o, zt.Tuning(method=WAVE, schedule=5, queue_paths=queue, tile_order=order))
+        assert np.array_equal(mega, wave), (queue, order)
+    big = workloads.mesh_field(subdivisions=2, meshes=16, spp=64, width=256, height=160)
+    mega = zt.render(big.scene, big.options, zt.Tuning(method=MEGA, schedule=1))
+    wave = zt.render(big.scene, big.options, zt.Tuning(method=WAVE, schedule=5, queue_paths=1))
+    assert np.array_equal(mega, wave)
+
+
 def test_wavefront_max_depth_zero_and_single_sample():
     s = zt.parse(scene_text("cornell"))
     for depth, spp in [(0, 3), (5, 1)]:
