@@ -893,7 +893,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 // pixel trails the others; a pixel's samples still run in order from its own
 // stream, so the image is the megakernel's bits. Segments, bands and queue
 // cursors as in the one-launch bounce.
-template <bool Q, typename Src>
+template <bool Q, bool kClaims, typename Src>
 ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap, uint32_t refill, uint32_t rps) {
   extern __shared__ int lds[];
   const LaneStack stk = laneStack<64>(lds, cap, w.spill);  // workgroups of one wave
@@ -912,7 +912,7 @@ ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap,
   };
   bool done = n == 0;
   if (!done) bounds(home);
-  uint32_t used_up = bandsUsedUp(w);
+  uint32_t used_up = kClaims ? bandsUsedUp(w) : 0u;
   bool has = false, tracing = false, sh = false;
   uint32_t slot = 0, si = 0;
   WfSlot<float> sl{};
@@ -931,7 +931,7 @@ ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap,
       out[3 * (uint64_t)sl.pixel + 0] = sl.sum.x * inv_spp;
       out[3 * (uint64_t)sl.pixel + 1] = sl.sum.y * inv_spp;
       out[3 * (uint64_t)sl.pixel + 2] = sl.sum.z * inv_spp;
-      again = claimPixelBand<float, false>(w, sl, home, used_up);
+      if constexpr (kClaims) again = claimPixelBand<float, false>(w, sl, home, used_up);
     }
     if (again) {
       storePixelSlot(w.slots, slot, sl);
@@ -1012,15 +1012,15 @@ ZT_D void wfPassBody(const Src& src, const WfShared<float>& w, int cur, int cap,
   if (overflow) atomicOr((int*)&w.counters[4], 1);
 }
 
-template <bool Q>
+template <bool Q, bool kClaims>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfPassOne(
     WfOne src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
-  wfPassBody<Q>(src, w, cur, cap, refill, rps);
+  wfPassBody<Q, kClaims>(src, w, cur, cap, refill, rps);
 }
-template <bool Q>
+template <bool Q, bool kClaims>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void wfPassMany(
     WfMany src, WfShared<float> w, int cur, int cap, uint32_t refill, uint32_t rps) {
-  wfPassBody<Q>(src, w, cur, cap, refill, rps);
+  wfPassBody<Q, kClaims>(src, w, cur, cap, refill, rps);
 }
 
 // The tail of a frame: once every pixel has been claimed and the live paths
@@ -1398,8 +1398,14 @@ void runWavefront(zt_ctx* ctx, const std::vector<WfScene<R>>& scenes, const zt_g
     sfn = single ? (quant ? (const void*)wfStepOne<true> : (const void*)wfStepOne<false>)
                  : (quant ? (const void*)wfStepMany<true> : (const void*)wfStepMany<false>);
   } else if (pass) {
-    sfn = single ? (quant ? (const void*)wfPassOne<true> : (const void*)wfPassOne<false>)
-                 : (quant ? (const void*)wfPassMany<true> : (const void*)wfPassMany<false>);
+    // A pool that holds every pixel never claims another: the passes are
+    // compiled without the claim path.
+    if (seg_slots >= (claims + kSegs - 1) / kSegs)
+      sfn = single ? (quant ? (const void*)wfPassOne<true, false> : (const void*)wfPassOne<false, false>)
+                   : (quant ? (const void*)wfPassMany<true, false> : (const void*)wfPassMany<false, false>);
+    else
+      sfn = single ? (quant ? (const void*)wfPassOne<true, true> : (const void*)wfPassOne<false, true>)
+                   : (quant ? (const void*)wfPassMany<true, true> : (const void*)wfPassMany<false, true>);
   }
   if (sfn) g_step = (int)residentBlocks(sfn, 64, wlds);
   // The tail hand-off (wfDrain): once every pixel is claimed and no more
