@@ -358,7 +358,7 @@ def main() -> int:
             k64.append(ctx.stats()["kernel_ms"])
         ctx.free(d64)
         fp64 = {"mpaths_s": round(paths / float(np.median(k64)) / 1e3, 2), "kernel_ms_median": round(float(np.median(k64)), 2),
-                "steps": 2, "method": "megakernel, one pixel per lane, two-child BVH"}
+                "steps": 2, "method": "wave-scheduled megakernel, two-child BVH walked in the reference's order"}
         log(f"[bench] fp64 verification path: {fp64}")
 
     cpu, ctr, rays_per_path, img_err = None, None, None, None

@@ -73,16 +73,19 @@ typedef struct zt_gpu_tuning {
   uint32_t queue_paths;   /* wavefront: path slots in flight (0 = min(pixels, 4M); sample passes: every pixel,
                              up to 64M; schedules 4 and 5 round up to 8 segments of whole 4096-slot ranges) */
   uint32_t variant;       /* megakernel build: 1 = registers unconstrained, 2/3/4 = compiled for >= 4/5/6 waves
-                             per SIMD; 0 = default (wave-scheduled: 2; one pixel per lane: fp32 4, fp64 1) */
+                             per SIMD (wave-scheduled over two-child nodes: 2/3/4 waves); 0 = default
+                             (wave-scheduled: four-wide 2, two-child fp32 4, fp64 3; one pixel per lane:
+                             fp32 4, fp64 1) */
   uint32_t stack_entries; /* traversal stack entries per lane kept in LDS, even, 8..64 (0 = wave-scheduled: 12;
                              one pixel per lane: fp32 8, fp64 16; wavefront: 12 for its wave-scheduled
                              tracer, else 16) */
   uint32_t bvh_width;     /* fp32 traversal: 0 or 4 = four-wide nodes when the scene has them, 2 = two-child
                              nodes (the fp64 path always walks the two-child nodes) */
   uint32_t schedule;      /* megakernel: 1 = one pixel per lane, every lane's ray finished before any lane shades;
-                             2 = wave-scheduled (fp32 over four-wide nodes): wave-uniform node / leaf passes,
-                             lanes shade and take new rays or pixels while other lanes keep traversing;
-                             0 = 2 where it applies, else 1.
+                             2 = wave-scheduled: wave-uniform node / leaf passes, lanes shade and take new
+                             rays or pixels while other lanes keep traversing (four-wide nodes, or two-child
+                             nodes walked in the reference's order: fp64 and fp32 with bvh_width 2);
+                             0 = 2 where it applies (not for a scene without a BVH), else 1.
                              wavefront: 1 = one ray per lane; 2 = the queue-fed tracer under the same
                              wave-uniform passes (fp32 over four-wide nodes), with shading and shadow
                              resolution in kernels of their own (four launches per bounce); 3 = that tracer
@@ -91,7 +94,8 @@ typedef struct zt_gpu_tuning {
                              bands; 5 = sample passes: the queue holds pixels and one launch runs a whole
                              sample of each; 0 = 5 where it applies, else 1 */
   uint32_t refill;        /* wave-scheduled: lanes waiting for work before the wave stops to serve them,
-                             1..64 (0 = 32; wavefront sample passes: 56 for one scene, 40 for a batch) */
+                             1..64 (0 = 32; fp64: 40; wavefront sample passes: 56 for one scene, 40 for a
+                             batch) */
   uint32_t node_bytes;    /* fp32 four-wide nodes: 0 or 128 = the fp32 boxes themselves, 64 = 64-byte nodes
                              (8-bit child boxes, decoded to boxes that contain the fp32 boxes) */
   uint32_t tile_order;    /* wave-scheduled megakernel: 1 = tiles handed out in scan order within each band,

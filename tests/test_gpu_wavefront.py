@@ -140,6 +140,36 @@ def test_wave_scheduled_megakernel_is_bit_identical():
         zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_w=2, tile_h=2))
 
 
+@pytest.mark.parametrize("precision,bvh_width", [(N.PRECISION_FP64, 0), (N.PRECISION_FP32, 2)])
+def test_wave_scheduled_two_child_walk_is_bit_identical(precision, bvh_width):
+    """Over the two-child tree (the fp64 path, or fp32 without the four-wide
+    tree) the wave-scheduled kernel walks each ray in the reference's order,
+    one step per pass, so it renders the one-pixel-per-lane kernel's bits for
+    any refill threshold, tile shape, build variant, stack capacity and tile
+    order; also with light sampling off and max_depth 0."""
+    from zigtrace_amd import workloads
+    wl = workloads.mesh_field(subdivisions=2, meshes=16, spp=3, width=70, height=37)
+    s, o = wl.scene, wl.options
+
+    def render(opts, **kw):
+        return zt.render(s, opts, zt.Tuning(method=MEGA, precision=precision, bvh_width=bvh_width, **kw))
+
+    base = render(o, schedule=1)
+    assert np.array_equal(base, render(o))  # the default form
+    for refill in (1, 17, 64):
+        for tw, th in ((8, 8), (13, 5), (64, 64)):
+            img = render(o, schedule=2, refill=refill, tile_w=tw, tile_h=th)
+            assert np.array_equal(base, img), (refill, tw, th)
+    for variant in (1, 2, 3, 4):
+        for cap in (8, 64):
+            img = render(o, schedule=2, variant=variant, stack_entries=cap)
+            assert np.array_equal(base, img), (variant, cap)
+    for order in (1, 2):
+        assert np.array_equal(base, render(o, schedule=2, tile_order=order)), order
+    for opts in (zt.Options(33, 20, 2, 12, light_sampling=False, seed=4), zt.Options(9, 7, 2, 0)):
+        assert np.array_equal(render(opts, schedule=1), render(opts, schedule=2))
+
+
 def test_two_child_and_four_wide_traversal_agree():
     """fp32 renders walk the four-wide BVH by default; the two-child walk finds the
     same closest primitives (exact t ties aside), so the images agree; fp64 always
@@ -189,6 +219,12 @@ def test_autotune_records_a_configuration_with_identical_output():
         # AUTO (and no tuning at all) now renders with the record: same bits.
         assert np.array_equal(ctx.render(sid, o, zt.Tuning()), default)
         assert np.array_equal(ctx.render(sid, o, None), default)
+        # fp64: the pixel-per-lane and wave-scheduled candidates render the same bits.
+        f64 = N.PRECISION_FP64
+        default64 = ctx.render(sid, o, zt.Tuning(method=MEGA, precision=f64))
+        tu, ms = ctx.autotune(sid, o, method=MEGA, precision=f64, probe_spp=1)
+        assert ms > 0 and tu.precision == f64
+        assert np.array_equal(ctx.render(sid, o, tu), default64), tu
     finally:
         ctx.close()
 

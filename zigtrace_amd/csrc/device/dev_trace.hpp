@@ -371,12 +371,15 @@ ZT_D HitRec<R> bruteForce(const SceneView<R>& sc, V3<R> o, V3<R> d, R tmin, R tm
 // The query a walk answers; kept by the caller (the wave-scheduled kernel
 // rebuilds it from the path state), so the walk itself holds only what
 // changes.
-struct Query4 {
-  V3<float> o, d;
-  float tmin;
+// A ray query of the resumable walkers (Trav4, Trav2).
+template <typename R>
+struct QueryR {
+  V3<R> o, d;
+  R tmin;
   int self, excl;
   bool anyhit;
 };
+using Query4 = QueryR<float>;
 
 struct Trav4 {
   V3<float> inv;
@@ -556,6 +559,101 @@ ZT_D HitRec<float> traverse4(const SceneView<float>& sc, const LaneStack& stk, V
   const Query4 q{o, d, tmin, self, excl, anyhit};
   return sc.nodes4q ? traverse4Impl<true>(sc, stk, q, tmax, overflow) : traverse4Impl<false>(sc, stk, q, tmax, overflow);
 }
+
+// The two-child walk of `traverse` (bvh.zig:80-126), resumable one step at a
+// time for the wave-scheduled kernels: the same entries are popped, pruned
+// and tested in the same order with the same closest distance, so the hit,
+// ties included, is the one `traverse` returns. A visited node's nearer hit
+// child is held rather than pushed and popped straight away; pruning could
+// not drop it (its entry distance is at most the closest hit). The tree must
+// have a root (num_nodes > 0).
+template <typename R>
+struct Trav2 {
+  V3<R> inv;
+  R closest;
+  int best, cur, sp, nsp;
+  bool found;
+
+  ZT_D void start(const QueryR<R>& q, R tmax) {
+    inv = mk(R(1) / q.d.x, R(1) / q.d.y, R(1) / q.d.z);
+    closest = tmax;
+    best = -1;
+    found = false;
+    sp = 0;
+    nsp = 0;
+    cur = R(entryT<R>(q.tmin)) > tmax ? kNoChild : 0;  // the root, pruned as `traverse` prunes it
+  }
+  ZT_D bool atLeaf() const { return cur < 0 && cur != kNoChild; }
+  ZT_D void push(const LaneStack& stk, int c, R t, int* overflow) {
+    if (sp == stk.cap) {
+      const int half = stk.cap >> 1;
+      if (nsp + half > kSpill) {
+        *overflow = 1;
+        return;
+      }
+      for (int i = 0; i < half; ++i) {
+        stk.scode[(size_t)(nsp + i) * stk.sstride] = stk.code[i * stk.stride];
+        stk.stval[(size_t)(nsp + i) * stk.sstride] = stk.tval[i * stk.stride];
+        stk.code[i * stk.stride] = stk.code[(i + half) * stk.stride];
+        stk.tval[i * stk.stride] = stk.tval[(i + half) * stk.stride];
+      }
+      nsp += half;
+      sp = half;
+    }
+    stk.code[sp * stk.stride] = c;
+    stk.tval[sp * stk.stride] = entryT<R>(t);
+    ++sp;
+  }
+  // Pops until an entry survives pruning (when none is held), then visits it
+  // if it is a node. Returns true when the tree is exhausted; a leaf is held
+  // for the leaf step.
+  ZT_D bool inner(const SceneView<R>& sc, const LaneStack& stk, const QueryR<R>& q, int* overflow) {
+    while (cur == kNoChild) {
+      if (sp == 0) {
+        if (nsp == 0) return true;
+        const int half = stk.cap >> 1;
+        nsp -= half;
+        for (int i = 0; i < half; ++i) {
+          stk.code[i * stk.stride] = stk.scode[(size_t)(nsp + i) * stk.sstride];
+          stk.tval[i * stk.stride] = stk.stval[(size_t)(nsp + i) * stk.sstride];
+        }
+        sp = half;
+      }
+      --sp;
+      if (!(R(stk.tval[sp * stk.stride]) > closest)) cur = stk.code[sp * stk.stride];
+    }
+    if (cur < 0) return false;
+    const DNode n = sc.nodes[cur];
+    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    R tl, tr;
+    const bool hl = slab<R>(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, q.o, inv, nx, ny, nz, q.tmin, closest, &tl) &&
+                    n.d.x != kNoChild;
+    const bool hr = slab<R>(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, q.o, inv, nx, ny, nz, q.tmin, closest, &tr) &&
+                    n.d.y != kNoChild;
+    if (hl && hr) {
+      const bool near_left = tl <= tr;
+      push(stk, near_left ? n.d.y : n.d.x, near_left ? tr : tl, overflow);
+      cur = near_left ? n.d.x : n.d.y;
+    } else {
+      cur = hl ? n.d.x : (hr ? n.d.y : kNoChild);
+    }
+    return false;
+  }
+  // Tests the held leaf. Returns true when an any-hit query found its hit.
+  ZT_D bool leaf(const SceneView<R>& sc, const QueryR<R>& q) {
+    const int code = cur;
+    cur = kNoChild;
+    if (leafHits<R>(sc, code, q.o, q.d, q.tmin, &closest, &best, q.self, q.excl, q.anyhit)) {
+      found = true;
+      return true;
+    }
+    return false;
+  }
+  ZT_D HitRec<R> finish(const SceneView<R>& sc, const QueryR<R>& q) {
+    if (!found) planeHits<R>(sc, q.o, q.d, q.tmin, &closest, &best, q.self, q.excl, q.anyhit);
+    return HitRec<R>{closest, best};
+  }
+};
 
 // One ray's closest-hit (or any-hit) query (scene.zig:108-116): brute force
 // when the scene has no BVH, the four-wide BVH on the fp32 path when the view

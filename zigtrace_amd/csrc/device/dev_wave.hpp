@@ -1,5 +1,5 @@
-// Wave-level pieces of the kernels that walk the four-wide BVH with one ray
-// per lane under wave-uniform passes (the wave-scheduled megakernel). `Src`
+// Wave-level pieces of the kernels that walk a BVH with one ray per lane
+// under wave-uniform passes (the wave-scheduled megakernel). `Src`
 // gives the scene view of a lane's scene index, `src.view(si)`, and says
 // whether every lane renders the same scene (`Src::kSingle`). The
 // workgroup must be exactly one wave: the leaf pass orders its LDS traffic
@@ -93,6 +93,19 @@ ZT_D bool wavePass(const Src& src, uint32_t si, Trav4& tr, const Query4& q, bool
                    uint8_t* owner_of) {
   if (2 * n_leaf >= n_node) return leafPassWave(src, si, tr, q, at_leaf, keys, owner_of);
   if (tracing && !at_leaf) return tr.template inner<Q>(src.view(si), stk, q, overflow);
+  return false;
+}
+
+// The same passes over two-child walkers (Trav2), with the same rule for a
+// leaf pass; each held leaf is tested in its own lane (the tree's leaves are
+// small, at most 4 primitives below the depth limit, and tested in the
+// reference's order).
+template <bool Q, typename R, typename Src>
+ZT_D bool wavePass(const Src& src, uint32_t si, Trav2<R>& tr, const QueryR<R>& q, bool tracing, bool at_leaf,
+                   uint32_t n_leaf, uint32_t n_node, const LaneStack& stk, int* overflow, unsigned long long*,
+                   uint8_t*) {
+  if (2 * n_leaf >= n_node) return at_leaf && tr.leaf(src.view(si), q);
+  if (tracing && !at_leaf) return tr.inner(src.view(si), stk, q, overflow);
   return false;
 }
 

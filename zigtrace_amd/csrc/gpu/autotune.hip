@@ -71,19 +71,21 @@ static std::vector<zt_gpu_tuning> candidates(uint32_t method, uint32_t precision
           c.push_back(tu);
         }
     for (auto& tu : c) tu.schedule = 1;  // the forms above: one pixel per lane
-    if (precision == ZT_PRECISION_FP32)  // wave-scheduled forms (four-wide scenes)
-      for (uint32_t variant : {1u, 2u})
-        for (uint32_t refill : {24u, 32u, 40u})
-          for (uint32_t cap : {8u, 12u, 16u}) {
-            zt_gpu_tuning tu{};
-            tu.method = method;
-            tu.precision = precision;
-            tu.variant = variant;
-            tu.stack_entries = cap;
-            tu.schedule = 2;
-            tu.refill = refill;
-            c.push_back(tu);
-          }
+    // Wave-scheduled forms: fp32 over the four-wide tree, fp64 over the
+    // two-child tree (where variants 2-4 target 2-4 waves per SIMD).
+    const bool f32 = precision == ZT_PRECISION_FP32;
+    for (uint32_t variant : f32 ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{2u, 3u, 4u})
+      for (uint32_t refill : {24u, 32u, 40u})
+        for (uint32_t cap : f32 ? std::vector<uint32_t>{8u, 12u, 16u} : std::vector<uint32_t>{12u}) {
+          zt_gpu_tuning tu{};
+          tu.method = method;
+          tu.precision = precision;
+          tu.variant = variant;
+          tu.stack_entries = cap;
+          tu.schedule = 2;
+          tu.refill = refill;
+          c.push_back(tu);
+        }
   } else {
     std::vector<uint32_t> pools;
     for (uint64_t q : {1ull << 20, 1ull << 21, 1ull << 22}) {

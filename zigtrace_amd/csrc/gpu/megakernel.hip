@@ -114,10 +114,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
   megakernelBody<R>(sc, pr, out, tile_w, tile_h, tiles_x, ntiles, cap, spill, band_next, err);
 }
 
-// Wave-scheduled megakernel of the fp32 path over the four-wide BVH. One
-// wave per workgroup; every lane owns a pixel (its stream, accumulator and
-// sample count, as renderPixel) and at most one ray in flight, walked with
-// the resumable Trav4. The wave runs wave-uniform passes -- node steps, leaf
+// Wave-scheduled megakernel. One wave per workgroup; every lane owns a pixel
+// (its stream, accumulator and sample count, as renderPixel) and at most one
+// ray in flight, walked with a resumable walker (Trav4 over the four-wide
+// tree of the fp32 path, Trav2 over the two-child tree). The wave runs wave-uniform passes -- node steps, leaf
 // tests -- until `refill` lanes are waiting for work, then those lanes shade
 // their hit (or apply their shadow ray), start their next ray or sample, or
 // take the next pixel of the wave's current tile, while the other lanes keep
@@ -127,19 +127,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
 // The scenes a wave-scheduled launch renders, with their tiles numbered in
 // one index space. One scene: everything in kernel arguments. Many scenes:
 // a table in device memory, each lane reading its own pixel's scene.
-struct OneScene {
+template <typename R>
+struct OneSceneT {
   static constexpr bool kSingle = true;
-  SceneView<float> sc;
+  SceneView<R> sc;
   Params pr;
-  float* out;
+  R* out;
   uint32_t tiles_x;
   ZT_D uint32_t sceneOf(uint32_t) const { return 0; }
   ZT_D uint32_t firstTile(uint32_t) const { return 0; }
   ZT_D uint32_t tilesX(uint32_t) const { return tiles_x; }
-  ZT_D const SceneView<float>& view(uint32_t) const { return sc; }
+  ZT_D const SceneView<R>& view(uint32_t) const { return sc; }
   ZT_D const Params& params(uint32_t) const { return pr; }
-  ZT_D float* output(uint32_t) const { return out; }
+  ZT_D R* output(uint32_t) const { return out; }
 };
+using OneScene = OneSceneT<float>;
 
 struct ManyScenes {
   static constexpr bool kSingle = false;
@@ -163,8 +165,9 @@ struct ManyScenes {
 
 // `order` (may be null) lists the tiles in the order they are handed out
 // within each band; the cost probe (kCost) instead counts the rays it starts
-// per tile into `cost`.
-template <bool Q, bool kCost, typename Src>
+// per tile into `cost`. `Walk` is the lanes' walker: Trav4 over the
+// four-wide tree (fp32), Trav2<R> over the two-child tree.
+template <typename R, typename Walk, bool Q, bool kCost, typename Src>
 ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap,
                              uint32_t refill, int* spill, uint32_t* band_next, const uint32_t* order,
                              uint32_t* cost, int* err) {
@@ -184,9 +187,9 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   uint32_t si = 0, x = 0, y = 0, sample = 0;
   int phase = kCamera;
   Rng rng{};
-  V3<float> sum = splat(0.0f);
-  Path<float> st;
-  Trav4 tr;
+  V3<R> sum = splat(R(0));
+  Path<R> st;
+  Walk tr;
   int overflow = 0;
   uint64_t n_ext = 0, n_sh = 0;  // rays started by the wave (wave-uniform)
   for (;;) {
@@ -227,7 +230,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
           y = py;
           has = true;
           rng = rngForStream(tp.seed, (uint64_t)py * tp.width + px);  // render.zig:135
-          sum = splat(0.0f);
+          sum = splat(R(0));
           sample = 0;
           phase = kCamera;
         }
@@ -236,12 +239,12 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
     // Lanes with no ray in flight advance their path to its next ray.
     const bool idle = has && !tracing;
     if (idle) {
-      const SceneView<float>& sc = src.view(si);
+      const SceneView<R>& sc = src.view(si);
       const Params& pr = src.params(si);
       for (;;) {
         if (phase == kResolve) {
-          const HitRec<float> hit{tr.closest, tr.best};  // what tr.finish left
-          phase = sh ? afterShadow<float>(hit, st) : shade<float>(sc, pr, hit, rng, st);
+          const HitRec<R> hit{tr.closest, tr.best};  // what tr.finish left
+          phase = sh ? afterShadow<R>(hit, st) : shade<R>(sc, pr, hit, rng, st);
         }
         if (phase == kFinish) {
           if (isFinite(st.radiance)) sum = add(sum, st.radiance);  // render.zig:141-142
@@ -250,21 +253,21 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
         }
         if (phase == kCamera) {
           if (sample == pr.spp) {
-            float* out = src.output(si);
+            R* out = src.output(si);
             const uint64_t pix = (uint64_t)y * pr.width + x;
-            const float inv_spp = 1.0f / (float)pr.spp;
+            const R inv_spp = R(1) / R(pr.spp);
             out[3 * pix + 0] = sum.x * inv_spp;
             out[3 * pix + 1] = sum.y * inv_spp;
             out[3 * pix + 2] = sum.z * inv_spp;
             has = false;
             break;
           }
-          phase = beginPath<float>(sc, pr, x, y, rng, st);
+          phase = beginPath<R>(sc, pr, x, y, rng, st);
           continue;  // kFinish at once when max_depth == 0
         }
         sh = phase == kShadow;
-        tr.start(Query4{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh},
-                 sh ? st.sh_tmax : INFINITY);
+        tr.start(QueryR<R>{st.ro, sh ? st.sh_d : st.rd, R(kRayEpsilon), st.self, sh ? st.sh_excl : -1, sh},
+                 sh ? st.sh_tmax : R(INFINITY));
         tracing = true;
         if constexpr (kCost)
           atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], 1u);
@@ -284,7 +287,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
     // Traverse in wave-uniform passes (wavePass) until enough lanes wait for
     // work.
     for (;;) {
-      const Query4 q{st.ro, sh ? st.sh_d : st.rd, (float)kRayEpsilon, st.self, sh ? st.sh_excl : -1, sh};
+      const QueryR<R> q{st.ro, sh ? st.sh_d : st.rd, R(kRayEpsilon), st.self, sh ? st.sh_excl : -1, sh};
       const bool at_leaf = tracing && tr.atLeaf();
       const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
       const uint32_t n_node = (uint32_t)__popcll(__ballot(tracing && !at_leaf));
@@ -309,7 +312,18 @@ template <int W, bool Q, bool kCost = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave(
     OneScene src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
     uint32_t* band_next, const uint32_t* order, uint32_t* cost, int* err) {
-  megakernelWaveBody<Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost, err);
+  megakernelWaveBody<float, Trav4, Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost,
+                                            err);
+}
+
+// The wave-scheduled kernel over the two-child tree: the fp64 path's default
+// (and fp32's when the scene keeps only the two-child tree).
+template <typename R, int W, bool kCost = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWave2(
+    OneSceneT<R> src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
+    uint32_t* band_next, const uint32_t* order, uint32_t* cost, int* err) {
+  megakernelWaveBody<R, Trav2<R>, false, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order,
+                                                cost, err);
 }
 
 // Many scenes in one persistent launch (zt_gpu_render_batch): the same
@@ -318,7 +332,8 @@ template <int W, bool Q, bool kCost = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void megakernelWaveBatch(
     ManyScenes src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap, uint32_t refill, int* spill,
     uint32_t* band_next, const uint32_t* order, uint32_t* cost, int* err) {
-  megakernelWaveBody<Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost, err);
+  megakernelWaveBody<float, Trav4, Q, kCost>(src, tile_w, tile_h, ntiles, cap, refill, spill, band_next, order, cost,
+                                            err);
 }
 
 __global__ void clearBands(uint32_t* band_next) {
@@ -484,26 +499,79 @@ void checkOptions(const zt_render_options* o) {
     throw Error(ZT_ERR_INVALID_ARGUMENT, "image size or spp beyond the device limits");
 }
 
+// Launch shape of the wave-scheduled kernels, from the tuning (0 = default).
+struct WaveShape {
+  uint32_t tw, th, refill, variant;
+  int cap;
+};
+static WaveShape waveShape(const zt_gpu_tuning& tu) {
+  // A tile is a wave's unit of work: 8x8 gives every lane one pixel to start
+  // with and keeps enough tiles for every resident wave on small images.
+  WaveShape w{tu.tile_w ? tu.tile_w : 8, tu.tile_h ? tu.tile_h : 8, tu.refill ? tu.refill : 32,
+              tu.variant ? tu.variant : 2u, tu.stack_entries ? (int)tu.stack_entries : 12};
+  if (w.tw * w.th < 64 || w.tw * w.th > 4096)
+    throw Error(ZT_ERR_INVALID_ARGUMENT, "wave-scheduled tiles must hold 64..4096 pixels");
+  if (w.refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
+  if (w.cap < 8 || w.cap > 64 || (w.cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
+  if (w.variant < 1 || w.variant > 4) throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+  if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
+  return w;
+}
+
+// One launch of a wave-scheduled kernel (every one takes its scene source
+// first, then these arguments), after clearing the band cursors.
+static void launchWave(zt_ctx* ctx, const void* f, uint32_t grid, void* src, const WaveShape& w, uint32_t nt,
+                       size_t lds, const uint32_t* ord, uint32_t* cost) {
+  uint32_t* bands = ctx->bands.as<uint32_t>();
+  int* spill = ctx->spill.as<int>();
+  int* err = ctx->errflag.as<int>();
+  uint32_t tw = w.tw, th = w.th, refill = w.refill;
+  int cap = w.cap;
+  void* args[] = {src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands,
+                  (void*)&ord, (void*)&cost, &err};
+  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
+  ZT_HIP(hipLaunchKernel(f, dim3(grid), dim3(64), args, lds, ctx->stream));
+  ctx->stats.launches += 2;
+}
+
+// Longest-first tile order: the cost probe `probe_fn` (one sample per pixel
+// without shadow rays: `probe_src` carries those parameters) counts the rays
+// each tile starts, and orderTiles sorts every band's tiles by that count,
+// most expensive first. Returns the order (context-owned).
+static const uint32_t* orderByProbe(zt_ctx* ctx, const void* probe_fn, uint32_t grid, void* probe_src,
+                                    const WaveShape& w, uint32_t nt, size_t lds) {
+  ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
+  ctx->tile_order.ensure((size_t)nt * sizeof(uint32_t));
+  uint32_t* cost = ctx->tile_cost.as<uint32_t>();
+  hipLaunchKernelGGL(zeroWords, dim3((nt + 255) / 256 < 1024 ? (nt + 255) / 256 : 1024), dim3(256), 0, ctx->stream,
+                     cost, nt);
+  launchWave(ctx, probe_fn, grid, probe_src, w, nt, lds, nullptr, cost);
+  hipLaunchKernelGGL(orderTiles, dim3(8), dim3(256), 0, ctx->stream, cost, ctx->tile_order.as<uint32_t>(), nt);
+  ctx->stats.launches += 2;
+  return ctx->tile_order.as<uint32_t>();
+}
+
+// Longest-first tile order (tuning tile_order: 0 = for one scene with at
+// least 16 samples per pixel, so the one-sample cost probe adds at most
+// 1/16, and more tiles than resident waves -- when every tile starts at once
+// their order cannot matter; 1 = scan order; 2 = always). A probe of one
+// pixel per 2x2 block, or with paths cut at depth 6, orders worse than it
+// saves; across the scenes of a batch the order mixes scenes within each
+// wave and loses more than the shorter tail gains.
+static bool orderTilesFor(const zt_gpu_tuning& tu, bool many, uint64_t pixels, uint64_t paths, uint64_t ntiles,
+                          uint32_t grid) {
+  return tu.tile_order == 2 || (tu.tile_order == 0 && !many && paths >= 16 * pixels && ntiles > grid);
+}
+
 // Wave-scheduled fp32 render over the four-wide BVH (see megakernelWaveBody)
 // of one scene (kernel arguments) or many (a scene table in device memory).
 void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning& tu) {
-  // A tile is a wave's unit of work: 8x8 gives every lane one pixel to start
-  // with and keeps enough tiles for every resident wave on small images.
-  const uint32_t tw = tu.tile_w ? tu.tile_w : 8;
-  const uint32_t th = tu.tile_h ? tu.tile_h : 8;
-  if (tw == 0 || th == 0 || tw * th < 64 || tw * th > 4096)
-    throw Error(ZT_ERR_INVALID_ARGUMENT, "wave-scheduled tiles must hold 64..4096 pixels");
-  const uint32_t refill = tu.refill ? tu.refill : 32;
-  if (refill > 64) throw Error(ZT_ERR_INVALID_ARGUMENT, "refill must be 1..64");
-  const int cap = tu.stack_entries ? (int)tu.stack_entries : 12;
-  if (cap < 8 || cap > 64 || (cap & 1)) throw Error(ZT_ERR_INVALID_ARGUMENT, "stack entries must be even, 8..64");
-  const uint32_t variant = tu.variant ? tu.variant : 2u;
-  if (variant < 1 || variant > 4) throw Error(ZT_ERR_INVALID_ARGUMENT, "megakernel variant must be 0..4");
+  const WaveShape w = waveShape(tu);
   uint64_t ntiles = 0;
   for (WaveJob& j : jobs) {
-    j.tiles_x = (j.pr.width + tw - 1) / tw;
+    j.tiles_x = (j.pr.width + w.tw - 1) / w.tw;
     j.first_tile = (uint32_t)ntiles;
-    ntiles += (uint64_t)j.tiles_x * ((j.pr.height + th - 1) / th);
+    ntiles += (uint64_t)j.tiles_x * ((j.pr.height + w.th - 1) / w.th);
   }
   if (ntiles == 0) return;
   if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
@@ -512,8 +580,7 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   const bool quant = jobs[0].view.nodes4q != nullptr;
   for (const WaveJob& j : jobs)
     if ((j.view.nodes4q != nullptr) != quant) throw Error(ZT_ERR_INTERNAL, "mixed node formats in one launch");
-  const uint32_t threads = 64;
-  const size_t lds = stackBytes(cap, (int)threads) + kLeafPassBytes;
+  const size_t lds = stackBytes(w.cap, 64) + kLeafPassBytes;
   static const void* const kOne[2][4] = {
       {(const void*)megakernelWave<1, false>, (const void*)megakernelWave<4, false>,
        (const void*)megakernelWave<5, false>, (const void*)megakernelWave<6, false>},
@@ -527,37 +594,19 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
   static const void* const kProbe[2][2] = {
       {(const void*)megakernelWave<4, false, true>, (const void*)megakernelWave<4, true, true>},
       {(const void*)megakernelWaveBatch<4, false, true>, (const void*)megakernelWaveBatch<4, true, true>}};
-  const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][variant - 1];
-  // Longest-first tile order (tuning tile_order: 0 = for one scene with at
-  // least 16 samples per pixel, so the one-sample cost probe adds at most
-  // 1/16, and more tiles than resident waves -- when every tile starts at
-  // once their order cannot matter; 1 = scan order; 2 = always): a
-  // one-sample render without shadow
-  // rays counts the rays each tile starts, orderTiles sorts every band by
-  // that count. (A probe of one pixel per 2x2 block, or with paths cut at
-  // depth 6, orders worse than it saves; across the scenes of a batch the
-  // order mixes scenes within each wave and loses more than the shorter
-  // tail gains.)
-  if (tu.tile_order > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "tile_order must be 0, 1 or 2");
+  const void* fn = (many ? kMany : kOne)[quant ? 1 : 0][w.variant - 1];
   uint64_t pixels = 0, paths = 0;
   for (const WaveJob& j : jobs) {
     pixels += (uint64_t)j.pr.width * j.pr.height;
     paths += (uint64_t)j.pr.width * j.pr.height * j.pr.spp;
   }
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
-  const bool ordered =
-      tu.tile_order == 2 || (tu.tile_order == 0 && !many && paths >= 16 * pixels && ntiles > grid);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, 64, lds));
+  const bool ordered = orderTilesFor(tu, many, pixels, paths, ntiles, grid);
   const void* probe_fn = kProbe[many ? 1 : 0][quant ? 1 : 0];
-  const uint32_t probe_grid =
-      ordered ? (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(probe_fn, threads, lds)) : 0;
-  ctx->spill.ensure(spillBytes((uint64_t)std::max(grid, probe_grid) * threads));
+  const uint32_t probe_grid = ordered ? (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(probe_fn, 64, lds)) : 0;
+  ctx->spill.ensure(spillBytes((uint64_t)std::max(grid, probe_grid) * 64));
   ctx->bands.ensure(8 * sizeof(uint32_t));
-  uint32_t* bands = ctx->bands.as<uint32_t>();
-  int* spill = ctx->spill.as<int>();
-  int* err = ctx->errflag.as<int>();
   const uint32_t nt = (uint32_t)ntiles;
-  const uint32_t* order = nullptr;
-  uint32_t* no_cost = nullptr;
   // One scene: everything in kernel arguments. Many: the scene table (and the
   // probe's copy of it, one sample per pixel) in device memory.
   std::vector<WaveJob> probe_jobs(jobs);
@@ -576,64 +625,69 @@ void launchWaveJobs(zt_ctx* ctx, std::vector<WaveJob>& jobs, const zt_gpu_tuning
     table = ManyScenes{ctx->jobs.as<WaveJob>(), (uint32_t)jobs.size()};
     table_probe = ManyScenes{ctx->jobs.as<WaveJob>() + jobs.size(), (uint32_t)jobs.size()};
   }
-  // All instantiations of a form take the same arguments.
-  auto launch = [&](const void* f, uint32_t g, bool probe, const uint32_t* ord, uint32_t* cost) {
-    void* src = many ? (void*)(probe ? &table_probe : &table) : (void*)(probe ? &one_probe : &one);
-    void* args[] = {src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands,
-                    (void*)&ord, (void*)&cost, &err};
-    hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
-    ZT_HIP(hipLaunchKernel(f, dim3(g), dim3(threads), args, lds, ctx->stream));
-    ctx->stats.launches += 2;
-  };
-  if (ordered) {
-    ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
-    ctx->tile_order.ensure((size_t)nt * sizeof(uint32_t));
-    uint32_t* cost = ctx->tile_cost.as<uint32_t>();
-    hipLaunchKernelGGL(zeroWords, dim3((nt + 255) / 256 < 1024 ? (nt + 255) / 256 : 1024), dim3(256), 0, ctx->stream,
-                       cost, nt);
-    launch(probe_fn, probe_grid, true, nullptr, cost);
-    hipLaunchKernelGGL(orderTiles, dim3(8), dim3(256), 0, ctx->stream, cost, ctx->tile_order.as<uint32_t>(), nt);
-    ctx->stats.launches += 2;
-    order = ctx->tile_order.as<uint32_t>();
-  }
-  launch(fn, grid, false, order, no_cost);
+  void* src = many ? (void*)&table : (void*)&one;
+  void* probe_src = many ? (void*)&table_probe : (void*)&one_probe;
+  const uint32_t* order = ordered ? orderByProbe(ctx, probe_fn, probe_grid, probe_src, w, nt, lds) : nullptr;
+  launchWave(ctx, fn, grid, src, w, nt, lds, order, nullptr);
+  ZT_HIP(hipGetLastError());
+}
+
+// Wave-scheduled render of one scene over its two-child tree (Trav2 walkers,
+// the reference's traversal order): the fp64 path's default. Variants 1-4
+// build for at least 1, 2, 3 or 4 waves per SIMD. Defaults measured on the
+// 1080p mesh field: fp64 variant 3 (168 VGPRs and 47 spilled: 3 waves beat
+// 2 without spills by 14%) with refill 40; fp32 variant 4 (128 VGPRs, no
+// spills) with refill 32.
+template <typename R>
+static void launchWaveTwoChild(zt_ctx* ctx, const SceneView<R>& view, const Params& pr, R* out,
+                               const zt_gpu_tuning& tu) {
+  WaveShape w = waveShape(tu);
+  if (!tu.variant) w.variant = K<R>::exact ? 3 : 4;
+  if (!tu.refill) w.refill = K<R>::exact ? 40 : 32;
+  const uint32_t tiles_x = (pr.width + w.tw - 1) / w.tw;
+  const uint64_t ntiles = (uint64_t)tiles_x * ((pr.height + w.th - 1) / w.th);
+  if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
+  static const void* const kFn[4] = {(const void*)megakernelWave2<R, 1>, (const void*)megakernelWave2<R, 2>,
+                                     (const void*)megakernelWave2<R, 3>, (const void*)megakernelWave2<R, 4>};
+  const void* fn = kFn[w.variant - 1];
+  const void* probe_fn = (const void*)megakernelWave2<R, 2, true>;
+  const size_t lds = stackBytes(w.cap, 64) + kLeafPassBytes;
+  const uint64_t pixels = (uint64_t)pr.width * pr.height;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, 64, lds));
+  const bool ordered = orderTilesFor(tu, false, pixels, pixels * pr.spp, ntiles, grid);
+  const uint32_t probe_grid = ordered ? (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(probe_fn, 64, lds)) : 0;
+  ctx->spill.ensure(spillBytes((uint64_t)std::max(grid, probe_grid) * 64));
+  ctx->bands.ensure(8 * sizeof(uint32_t));
+  const uint32_t nt = (uint32_t)ntiles;
+  Params probe = pr;  // one sample, extension rays only
+  probe.spp = 1;
+  probe.use_nee = 0;
+  OneSceneT<R> src{view, pr, out, tiles_x}, probe_src{view, probe, out, tiles_x};
+  const uint32_t* order = ordered ? orderByProbe(ctx, probe_fn, probe_grid, &probe_src, w, nt, lds) : nullptr;
+  launchWave(ctx, fn, grid, &src, w, nt, lds, order, nullptr);
   ZT_HIP(hipGetLastError());
 }
 
 const uint32_t* probeTileOrder(zt_ctx* ctx, const SceneView<float>& view, const Params& pr, float* out, int cap,
                                uint32_t refill) {
-  const uint32_t tw = 8, th = 8, threads = 64;
-  const uint32_t tiles_x = (pr.width + tw - 1) / tw;
-  const uint64_t ntiles = (uint64_t)tiles_x * ((pr.height + th - 1) / th);
+  zt_gpu_tuning tu{};
+  tu.stack_entries = (uint32_t)cap;
+  tu.refill = refill;
+  const WaveShape w = waveShape(tu);  // 8x8 tiles
+  const uint32_t tiles_x = (pr.width + w.tw - 1) / w.tw;
+  const uint64_t ntiles = (uint64_t)tiles_x * ((pr.height + w.th - 1) / w.th);
   if (ntiles > 0xFFFFFFFFull - 8) throw Error(ZT_ERR_INVALID_ARGUMENT, "too many tiles in one launch");
   const bool quant = view.nodes4q != nullptr;
   const void* fn = quant ? (const void*)megakernelWave<4, true, true> : (const void*)megakernelWave<4, false, true>;
-  const size_t lds = stackBytes(cap, (int)threads) + kLeafPassBytes;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, threads, lds));
-  ctx->spill.ensure(spillBytes((uint64_t)grid * threads));
+  const size_t lds = stackBytes(cap, 64) + kLeafPassBytes;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, residentBlocks(fn, 64, lds));
+  ctx->spill.ensure(spillBytes((uint64_t)grid * 64));
   ctx->bands.ensure(8 * sizeof(uint32_t));
-  const uint32_t nt = (uint32_t)ntiles;
-  ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
-  ctx->tile_order.ensure((size_t)nt * sizeof(uint32_t));
-  uint32_t* cost = ctx->tile_cost.as<uint32_t>();
-  uint32_t* bands = ctx->bands.as<uint32_t>();
-  int* spill = ctx->spill.as<int>();
-  int* err = ctx->errflag.as<int>();
   Params probe = pr;  // one sample, extension rays only
   probe.spp = 1;
   probe.use_nee = 0;
   OneScene src{view, probe, out, tiles_x};
-  const uint32_t* no_order = nullptr;
-  void* args[] = {(void*)&src, (void*)&tw, (void*)&th, (void*)&nt, (void*)&cap, (void*)&refill, &spill, &bands,
-                  (void*)&no_order, (void*)&cost, &err};
-  hipLaunchKernelGGL(zeroWords, dim3((nt + 255) / 256 < 1024 ? (nt + 255) / 256 : 1024), dim3(256), 0, ctx->stream,
-                     cost, nt);
-  hipLaunchKernelGGL(clearBands, dim3(1), dim3(64), 0, ctx->stream, bands);
-  ZT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, ctx->stream));
-  hipLaunchKernelGGL(orderTiles, dim3(8), dim3(256), 0, ctx->stream, cost, ctx->tile_order.as<uint32_t>(), nt);
-  ZT_HIP(hipGetLastError());
-  ctx->stats.launches += 4;
-  return ctx->tile_order.as<uint32_t>();
+  return orderByProbe(ctx, fn, grid, &src, w, (uint32_t)ntiles, lds);
 }
 
 uint32_t residentBlocks(const void* kernel, uint32_t threads, size_t lds) {
@@ -657,13 +711,21 @@ void launchMegakernel(zt_ctx* ctx, const DevScene& s, const zt_render_options& o
   constexpr bool kF64 = K<R>::exact;
   if (tu.schedule > 2) throw Error(ZT_ERR_INVALID_ARGUMENT, "schedule must be 0..2");
   int* err = ctx->errflag.as<int>();
-  if constexpr (!kF64) {
-    // fp32 over the four-wide tree: wave-scheduled unless asked otherwise.
-    // Defaults measured on the 1080p mesh field: a build for >= 4 waves per
-    // SIMD (128 VGPRs, no spills), 12 LDS stack entries, refill 32.
-    if (view.nodes4 && tu.schedule != 1) {
-      std::vector<WaveJob> jobs{WaveJob{view, pr, dout, 0, 0}};
-      launchWaveJobs(ctx, jobs, tu);
+  // Wave-scheduled unless asked otherwise. fp32 over the four-wide tree:
+  // defaults measured on the 1080p mesh field, a build for >= 4 waves per
+  // SIMD (128 VGPRs, no spills), 12 LDS stack entries, refill 32. Over the
+  // two-child tree (fp64, or fp32 without the four-wide tree) the same
+  // scheduling with the reference's traversal order.
+  if (tu.schedule != 1) {
+    if constexpr (!kF64) {
+      if (view.nodes4) {
+        std::vector<WaveJob> jobs{WaveJob{view, pr, dout, 0, 0}};
+        launchWaveJobs(ctx, jobs, tu);
+        return;
+      }
+    }
+    if (view.num_nodes > 0) {
+      launchWaveTwoChild<R>(ctx, view, pr, dout, tu);
       return;
     }
   }
