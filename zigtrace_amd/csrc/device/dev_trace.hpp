@@ -130,6 +130,27 @@ ZT_D bool slabNear(float nx, float ny, float nz, float fx, float fy, float fz, V
   *tout = lo;
   return !(hi < lo);
 }
+// The fp32 four-wide walker's form of slabNear: each plane distance is one
+// fma, p * inv - oi, with the ray's origin-over-direction oi = o * inv taken
+// from per-ray offsets shifted to cover its rounding (onear >= o * inv for
+// near planes, ofar <= o * inv for far planes), so a near distance is never
+// above the exact (p - o) * inv and a far distance never below it, up to the
+// fma's own rounding, which the far distance's widening covers as in
+// slabNear. Boxes the exact test hits are never lost.
+ZT_D bool slabFma(float nx, float ny, float nz, float fx, float fy, float fz, V3<float> inv, V3<float> onear,
+                  V3<float> ofar, float tmin, float tmax, float* tout) {
+  const float tx0 = __builtin_fmaf(nx, inv.x, -onear.x);
+  const float tx1 = __builtin_fmaf(fx, inv.x, -ofar.x);
+  const float ty0 = __builtin_fmaf(ny, inv.y, -onear.y);
+  const float ty1 = __builtin_fmaf(fy, inv.y, -ofar.y);
+  const float tz0 = __builtin_fmaf(nz, inv.z, -onear.z);
+  const float tz1 = __builtin_fmaf(fz, inv.z, -ofar.z);
+  const float lo = fmaxf(fmaxf(tmin, tx0), fmaxf(ty0, tz0));
+  const float far = fminf(tx1, fminf(ty1, tz1)) * (1.0f + 3.0f * 0x1.0p-23f);
+  const float hi = fminf(tmax, far);
+  *tout = lo;
+  return !(hi < lo);
+}
 template <typename R>
 ZT_D bool slab(float bx0, float by0, float bz0, float bx1, float by1, float bz1, V3<R> o, V3<R> inv, bool nx,
                bool ny, bool nz, R tmin, R tmax, R* tout) {
@@ -383,12 +404,19 @@ using Query4 = QueryR<float>;
 
 struct Trav4 {
   V3<float> inv;
+  V3<float> onear, ofar;  // o * inv rounded, raised / lowered by 2^-22 of itself (slabFma)
   float closest;
   int best, cur, sp, nsp;
   bool found;
 
   ZT_D void start(const Query4& q, float tmax) {
     inv = mk(1.0f / q.d.x, 1.0f / q.d.y, 1.0f / q.d.z);
+    // |fl(o * inv) - o * inv| <= 2^-24 |fl(o * inv)|; the shift (an exact
+    // power-of-two scaling) covers that and the rounding of the sum.
+    const V3<float> oi = mk(q.o.x * inv.x, q.o.y * inv.y, q.o.z * inv.z);
+    const V3<float> e = mk(fabsf(oi.x) * 0x1.0p-22f, fabsf(oi.y) * 0x1.0p-22f, fabsf(oi.z) * 0x1.0p-22f);
+    onear = mk(oi.x + e.x, oi.y + e.y, oi.z + e.z);
+    ofar = mk(oi.x - e.x, oi.y - e.y, oi.z - e.z);
     closest = tmax;
     best = -1;
     found = false;
@@ -496,10 +524,10 @@ struct Trav4 {
       c1 = ch.y;
       c2 = ch.z;
       c3 = ch.w;
-      h0 = slabNear<float>(ax.x, ay.x, az.x, fx.x, fy.x, fz.x, q.o, inv, q.tmin, closest, &k0);
-      h1 = slabNear<float>(ax.y, ay.y, az.y, fx.y, fy.y, fz.y, q.o, inv, q.tmin, closest, &k1);
-      h2 = slabNear<float>(ax.z, ay.z, az.z, fx.z, fy.z, fz.z, q.o, inv, q.tmin, closest, &k2);
-      h3 = slabNear<float>(ax.w, ay.w, az.w, fx.w, fy.w, fz.w, q.o, inv, q.tmin, closest, &k3);
+      h0 = slabFma(ax.x, ay.x, az.x, fx.x, fy.x, fz.x, inv, onear, ofar, q.tmin, closest, &k0);
+      h1 = slabFma(ax.y, ay.y, az.y, fx.y, fy.y, fz.y, inv, onear, ofar, q.tmin, closest, &k1);
+      h2 = slabFma(ax.z, ay.z, az.z, fx.z, fy.z, fz.z, inv, onear, ofar, q.tmin, closest, &k2);
+      h3 = slabFma(ax.w, ay.w, az.w, fx.w, fy.w, fz.w, inv, onear, ofar, q.tmin, closest, &k3);
     }
     // Misses (and empty slots, whose boxes are inverted) sort last.
     const bool v0 = h0 && c0 != kNoChild, v1 = h1 && c1 != kNoChild, v2 = h2 && c2 != kNoChild,
