@@ -479,7 +479,10 @@ struct Trav4 {
         sp = half;
       }
       --sp;
-      if (stk.tval[sp * stk.stride] <= closest) cur = stk.code[sp * stk.stride];
+      // Both words read together (one LDS round trip per pop, not two).
+      const float tv = stk.tval[sp * stk.stride];
+      const int code = stk.code[sp * stk.stride];
+      cur = tv <= closest ? code : kNoChild;
     }
     if (cur < 0) return false;
     float k0, k1, k2, k3;
@@ -648,7 +651,9 @@ struct Trav2 {
         sp = half;
       }
       --sp;
-      if (!(R(stk.tval[sp * stk.stride]) > closest)) cur = stk.code[sp * stk.stride];
+      const float tv = stk.tval[sp * stk.stride];
+      const int code = stk.code[sp * stk.stride];
+      cur = R(tv) > closest ? kNoChild : code;
     }
     if (cur < 0) return false;
     const DNode n = sc.nodes[cur];
