@@ -410,7 +410,15 @@ struct Trav4 {
   bool found;
 
   ZT_D void start(const Query4& q, float tmax) {
-    inv = mk(1.0f / q.d.x, 1.0f / q.d.y, 1.0f / q.d.z);
+    // A zero direction component (a draw of exactly 0.5 makes one, e.g. off
+    // an axis-aligned plane) has an infinite reciprocal, and o * inf minus
+    // itself would be NaN: that axis would stop culling and the ray walk most
+    // of the tree. Reciprocals are held to +-2^100 instead, which still culls
+    // every box the exact test culls but those whose face lies within
+    // 2^-100 * tmin of the origin on that axis.
+    inv = mk(fminf(fmaxf(1.0f / q.d.x, -0x1.0p100f), 0x1.0p100f),
+             fminf(fmaxf(1.0f / q.d.y, -0x1.0p100f), 0x1.0p100f),
+             fminf(fmaxf(1.0f / q.d.z, -0x1.0p100f), 0x1.0p100f));
     // |fl(o * inv) - o * inv| <= 2^-24 |fl(o * inv)|; the shift (an exact
     // power-of-two scaling) covers that and the rounding of the sum.
     const V3<float> oi = mk(q.o.x * inv.x, q.o.y * inv.y, q.o.z * inv.z);
