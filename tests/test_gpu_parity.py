@@ -66,6 +66,40 @@ def test_bvh_traversal_equals_brute_force(gpu_ctx, n, precision, mode):
         assert h.sum() > len(rays) // 4
 
 
+@pytest.mark.parametrize("precision", [N.PRECISION_FP32, N.PRECISION_FP64])
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_axis_parallel_rays_equal_brute_force(gpu_ctx, precision, mode):
+    """Directions with components of exactly +0 or -0 (a uniform draw of exactly
+    one half makes them, e.g. off an axis-aligned plane) have infinite
+    reciprocals, which the four-wide walker holds to +-2^100; every node format
+    must still find brute force's hits, in both precisions."""
+    rng = np.random.default_rng(77)
+    sc = random_scene(rng, 400)
+    sid = gpu_ctx.upload(sc)
+    shapes = sc.view_arrays()["shapes"]
+    rays = np.zeros(3000, dtype=N.RAY)
+    for k in range(len(rays)):
+        s = shapes[rng.integers(len(shapes))]
+        c = s["v"][:3] if s["kind"] == N.SPHERE else s["v"].reshape(3, 3).mean(0)
+        d = np.zeros(3)
+        axes = rng.permutation(3)[: 1 + k % 2]  # one axis, or a coordinate plane
+        d[axes] = rng.choice([-1.0, 1.0], len(axes)) * rng.uniform(0.2, 1.0, len(axes))
+        d /= np.linalg.norm(d)
+        o = c - 20.0 * d + np.where(d == 0, rng.uniform(-0.3, 0.3, 3), 0.0)
+        zero = d == 0
+        d[zero] = np.where(rng.random(zero.sum()) < 0.5, 0.0, -0.0)
+        rays[k]["origin"] = o
+        rays[k]["dir"] = d
+    bvh = gpu_ctx.intersect(sid, rays, 1e-6, 1e9, mode=mode, precision=precision)
+    brute = gpu_ctx.intersect(sid, rays, 1e-6, 1e9, mode=1, precision=precision)
+    gpu_ctx.release(sid)
+    assert np.array_equal(bvh["hit"], brute["hit"])
+    h = bvh["hit"] == 1
+    assert h.sum() > len(rays) // 4
+    assert np.array_equal(bvh["t"][h], brute["t"][h])
+    assert np.array_equal(bvh["material"][h], brute["material"][h])
+
+
 def test_fp64_intersect_equals_oracle(gpu_ctx):
     rng = np.random.default_rng(99)
     sc = random_scene(rng, 300, planes=2)
