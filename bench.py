@@ -28,7 +28,8 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
     against those oracle pixels (relative RMSE, fraction of pixels moved by a
     diverged path, extension/shadow ray counts of both),
   * the fp64 verification path's Mpaths/s on the same frame (the reference's
-    precision),
+    precision), and the plain megakernel's (one pixel per lane) with the
+    wavefront's rate over it,
   * the two smaller configurations (the shipped Cornell box at 400x400, 64 spp;
     the sphere field at 600x340, 16 spp), GPU beside the oracle.
 
@@ -347,7 +348,7 @@ def main() -> int:
     best = max(results, key=lambda k: results[k]["mpaths_s"])
     r = results[best]
 
-    fp64 = None
+    fp64, plain = None, None
     if not args.no_extra:  # the verification path (the reference's precision) on the same frame
         d64 = ctx.alloc(o.width * o.height * 3 * 8)
         tu64 = zt.Tuning(precision=N.PRECISION_FP64)
@@ -360,6 +361,18 @@ def main() -> int:
         fp64 = {"mpaths_s": round(paths / float(np.median(k64)) / 1e3, 2), "kernel_ms_median": round(float(np.median(k64)), 2),
                 "steps": 2, "method": "wave-scheduled megakernel, two-child BVH walked in the reference's order"}
         log(f"[bench] fp64 verification path: {fp64}")
+        # The plain megakernel (one pixel per lane, library defaults) on the same frame, beside the wavefront.
+        tu1 = zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=1)
+        ctx.render_device(sid, o, tu1, dout)
+        k1 = []
+        for _ in range(2):
+            ctx.render_device(sid, o, tu1, dout)
+            k1.append(ctx.stats()["kernel_ms"])
+        plain = {"mpaths_s": round(paths / float(np.median(k1)) / 1e3, 2), "kernel_ms_median": round(float(np.median(k1)), 2),
+                 "steps": 2, "method": "megakernel, one pixel per lane (schedule 1)"}
+        if "wavefront" in results:
+            plain["wavefront_over_plain"] = round(results["wavefront"]["mpaths_s"] / plain["mpaths_s"], 3)
+        log(f"[bench] plain megakernel: {plain}")
 
     cpu, ctr, rays_per_path, img_err = None, None, None, None
     if not args.no_cpu:
@@ -421,6 +434,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "image_error": img_err,
         "fp64_verification": fp64,
+        "plain_megakernel": plain,
         "library": N.loaded_library(),
         **extra,
     }
