@@ -117,13 +117,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W :
 // Wave-scheduled megakernel. One wave per workgroup; every lane owns a pixel
 // (its stream, accumulator and sample count, as renderPixel) and at most one
 // ray in flight, walked with a resumable walker (Trav4 over the four-wide
-// tree of the fp32 path, Trav2 over the two-child tree). The wave runs wave-uniform passes -- node steps, leaf
-// tests -- until `refill` lanes are waiting for work, then those lanes shade
-// their hit (or apply their shadow ray), start their next ray or sample, or
-// take the next pixel of the wave's current tile, while the other lanes keep
-// their traversal state. Each pixel's computation is the same sequence of
-// device functions as renderPixel's, so the image is identical; only the
-// interleaving of lanes changes.
+// tree of the fp32 path, Trav2 over the two-child tree). The wave runs
+// wave-uniform passes -- node steps, leaf tests -- until `refill` lanes are
+// waiting for work, then those lanes shade their hit (or apply their shadow
+// ray), start their next ray or sample, or take the next pixel of the wave's
+// current tile, while the other lanes keep their traversal state. Each
+// pixel's computation is the same sequence of device functions as
+// renderPixel's, so the image is identical; only the interleaving of lanes
+// changes.
 // The scenes a wave-scheduled launch renders, with their tiles numbered in
 // one index space. One scene: everything in kernel arguments. Many scenes:
 // a table in device memory, each lane reading its own pixel's scene.
