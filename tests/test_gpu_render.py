@@ -38,6 +38,32 @@ def test_empty_scene_renders_background_exactly():  # render.zig:188-196
     s.build()
     img = zt.render(s, opts())
     assert np.array_equal(img, np.broadcast_to(np.float32([0.25, 0.5, 0.75]).astype(np.float64), img.shape))
+    # Every form, both precisions: no tree, no primitive, no light.
+    for precision in (N.PRECISION_FP32, N.PRECISION_FP64):
+        want = np.float32([0.25, 0.5, 0.75]) if precision == N.PRECISION_FP32 else np.float64([0.25, 0.5, 0.75])
+        for tu in (zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=1), zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=2),
+                   zt.Tuning(method=N.METHOD_WAVEFRONT), zt.Tuning(method=N.METHOD_WAVEFRONT, schedule=4)):
+            tu.precision = precision
+            got = zt.render(s, opts(), tu)
+            assert np.array_equal(got, np.broadcast_to(want.astype(np.float64), got.shape)), (precision, tu)
+
+
+def test_planes_only_scene_is_the_same_in_every_form():
+    """A built scene of planes alone has no tree (planes are tested after it,
+    scene.zig:113-114); every form and schedule renders the same bits."""
+    s = zt.Scene()
+    s.background = zt.sky((1, 1, 1), (0.5, 0.7, 1.0))
+    m = s.add_material(zt.lambertian((0.6, 0.5, 0.4)))
+    s.add(zt.plane((0, -1, 0), (0, 1, 0), m))
+    s.camera = zt.CameraConfig((0, 0.5, 1), (0, 0, -3))
+    s.build()
+    for precision in (N.PRECISION_FP32, N.PRECISION_FP64):
+        base = zt.render(s, opts(), zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=1, precision=precision))
+        assert base.any()
+        for tu in (zt.Tuning(method=N.METHOD_MEGAKERNEL, schedule=2), zt.Tuning(method=N.METHOD_WAVEFRONT),
+                   zt.Tuning(method=N.METHOD_WAVEFRONT, schedule=4)):
+            tu.precision = precision
+            assert np.array_equal(base, zt.render(s, opts(), tu)), (precision, tu)
 
 
 def test_diffuse_sphere_under_uniform_sky_is_albedo():  # render.zig:230-243
