@@ -35,8 +35,9 @@ Reported beside `value` (which is the fastest method's whole-frame rate):
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method auto|megakernel|wavefront]
                        [--no-cpu] [--no-extra] [--no-tune] [--spp S]
-       python bench.py --only batch|adjoint|configs [--adjoint-steps S]   (one side leg alone: the
-                                             batch, the adjoint, or the two smaller configurations)
+       python bench.py --only batch|adjoint|configs|fp64 [--adjoint-steps S]   (one side leg alone:
+                                             the batch, the adjoint, the two smaller configurations, or
+                                             the fp64 verification path on the benchmark frame)
        python bench.py --copy-probe BYTES   (one device-to-device copy of BYTES, for
                                              calibrating HBM traffic counters)
 """
@@ -260,6 +261,23 @@ def bench_adjoint(zt, N, W, steps: int):
             "param_rel_err_init": round(err0, 4), "param_rel_err_final": round(err1, 4)}
 
 
+def bench_fp64(zt, N, W, spp: int) -> dict:
+    """The fp64 verification path (library defaults) on the benchmark frame: one
+    render after a warm-up; Mpaths/s from the device's kernel time."""
+    w = W.mesh_field(spp=spp)
+    o = w.options
+    ctx = zt.Context(0)
+    sid = ctx.upload(w.scene)
+    d64 = ctx.alloc(o.width * o.height * 3 * 8)
+    tu = zt.Tuning(precision=N.PRECISION_FP64)
+    ctx.render_device(sid, o, tu, d64)
+    ctx.render_device(sid, o, tu, d64)
+    ms = ctx.stats()["kernel_ms"]
+    ctx.free(d64)
+    ctx.close()
+    return {"mpaths_s": round(o.width * o.height * o.spp / ms / 1e3, 2), "kernel_ms": round(ms, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,12 +289,14 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner")
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel (64 is the benchmark; less for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
-    ap.add_argument("--only", choices=["batch", "adjoint", "configs"], default=None,
-                    help="run one side leg alone and print its record (for profiling its kernels)")
+    ap.add_argument("--only", choices=["batch", "adjoint", "configs", "fp64"], default=None,
+                    help="run one side leg alone and print its record (for profiling its kernels); fp64: the "
+                         "verification path on the benchmark frame, library defaults, one render after a warm-up")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="Adam steps of the adjoint leg")
     ap.add_argument("--schedule", type=int, default=0,
-                    help="schedule with --no-tune (megakernel: 0/1 pixel per lane, 2 wave-scheduled; wavefront: "
-                         "0/2 four-launch bounce, 1 one ray per lane, 3 two-launch bounce)")
+                    help="schedule with --no-tune (megakernel: 1 one pixel per lane, 0/2 wave-scheduled; wavefront: "
+                         "1 one ray per lane, 2 four-launch bounce, 3 two-launch bounce, 4 one-launch bounce, "
+                         "0/5 sample passes)")
     ap.add_argument("--refill", type=int, default=0, help="wave-scheduled refill threshold with --no-tune")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries with --no-tune")
     args = ap.parse_args()
@@ -298,6 +318,8 @@ def main() -> int:
             leg = bench_batch(zt, N, W, steps=1)
         elif args.only == "adjoint":
             leg = bench_adjoint(zt, N, W, args.adjoint_steps)
+        elif args.only == "fp64":
+            leg = bench_fp64(zt, N, W, args.spp)
         else:
             leg = bench_configs(zt, N, W)
         print(json.dumps({args.only: leg}), flush=True)
