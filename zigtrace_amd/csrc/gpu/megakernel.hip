@@ -193,6 +193,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
   Walk tr;
   int overflow = 0;
   uint64_t n_ext = 0, n_sh = 0;  // rays started by the wave (wave-uniform)
+  uint32_t pix_rays = 0;          // cost probe: rays the lane's pixel started
   for (;;) {
     // Pixels for lanes without one, from the wave's tile (then the next tile).
     for (;;) {
@@ -260,6 +261,10 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
             out[3 * pix + 0] = sum.x * inv_spp;
             out[3 * pix + 1] = sum.y * inv_spp;
             out[3 * pix + 2] = sum.z * inv_spp;
+            if constexpr (kCost) {  // one add per pixel, not one per ray
+              atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], pix_rays);
+              pix_rays = 0;
+            }
             has = false;
             break;
           }
@@ -270,8 +275,7 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
         tr.start(QueryR<R>{st.ro, sh ? st.sh_d : st.rd, R(kRayEpsilon), st.self, sh ? st.sh_excl : -1, sh},
                  sh ? st.sh_tmax : R(INFINITY));
         tracing = true;
-        if constexpr (kCost)
-          atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], 1u);
+        if constexpr (kCost) ++pix_rays;
         break;
       }
     }
