@@ -165,8 +165,8 @@ struct ManyScenes {
 };
 
 // `order` (may be null) lists the tiles in the order they are handed out
-// within each band; the cost probe (kCost) instead counts the rays it starts
-// per tile into `cost`. `Walk` is the lanes' walker: Trav4 over the
+// within each band; the cost probe (kCost) instead adds up per tile, in
+// `cost`, the cube of the rays each of the tile's pixels starts. `Walk` is the lanes' walker: Trav4 over the
 // four-wide tree (fp32), Trav2<R> over the two-child tree.
 template <typename R, typename Walk, bool Q, bool kCost, typename Src>
 ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, uint32_t ntiles, int cap,
@@ -262,7 +262,13 @@ ZT_D void megakernelWaveBody(const Src& src, uint32_t tile_w, uint32_t tile_h, u
             out[3 * pix + 1] = sum.y * inv_spp;
             out[3 * pix + 2] = sum.z * inv_spp;
             if constexpr (kCost) {  // one add per pixel, not one per ray
-              atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], pix_rays);
+              // The pixel's rays cubed: a tile holding a few long pixels ranks
+              // above one whose cost is spread evenly, so the long chains that
+              // end the frame start early. Below 2^20 per pixel, a tile's sum
+              // (at most 4096 pixels) stays below 2^32.
+              const uint64_t c = pix_rays;
+              const uint32_t weight = (uint32_t)min(c * c * c, (uint64_t)((1u << 20) - 1u));
+              atomicAdd(&cost[src.firstTile(si) + (y / tile_h) * src.tilesX(si) + x / tile_w], weight);
               pix_rays = 0;
             }
             has = false;
@@ -540,9 +546,9 @@ static void launchWave(zt_ctx* ctx, const void* f, uint32_t grid, void* src, con
 }
 
 // Longest-first tile order: the cost probe `probe_fn` (one sample per pixel
-// without shadow rays: `probe_src` carries those parameters) counts the rays
-// each tile starts, and orderTiles sorts every band's tiles by that count,
-// most expensive first. Returns the order (context-owned).
+// without shadow rays: `probe_src` carries those parameters) sums over each
+// tile the cube of its pixels' ray counts, and orderTiles sorts every band's
+// tiles by that sum, most expensive first. Returns the order (context-owned).
 static const uint32_t* orderByProbe(zt_ctx* ctx, const void* probe_fn, uint32_t grid, void* probe_src,
                                     const WaveShape& w, uint32_t nt, size_t lds) {
   ctx->tile_cost.ensure((size_t)nt * sizeof(uint32_t));
