@@ -248,6 +248,43 @@ def test_longest_first_tile_order_is_bit_identical():
         zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_order=3))
 
 
+MIRROR_BOX = """
+image 64 64
+samples 2
+depth 400
+camera from 0 0 3 to 0 0 -1 fov 70
+background color 0 0 0
+material mirror metal 1 1 1 0
+material lamp emissive 4 4 4
+quad -5 -5 -5  10 0 0  0 10 0  mirror
+quad -5 -5 5   10 0 0  0 10 0  mirror
+quad -5 -5 -5  0 10 0  0 0 10  mirror
+quad 5 -5 -5   0 10 0  0 0 10  mirror
+quad -5 -5 -5  10 0 0  0 0 10  mirror
+quad -5 5 -5   10 0 0  0 0 10  mirror
+sphere 2 2 -3 0.7 lamp
+"""
+
+
+def test_tile_order_with_deep_paths_and_the_largest_tiles():
+    """A closed box of perfect mirrors: no path escapes, roulette keeps 95% per
+    bounce, so with depth 400 some one-sample probe paths start more than 100
+    rays and their cubed counts reach the per-pixel cap; with 64x64 tiles a
+    tile's sum of 4,096 capped terms must still fit its 32-bit counter. Forced
+    longest-first order gives the same bits as scan order and the one-pixel-
+    per-lane kernel, in both precisions."""
+    s = zt.parse(MIRROR_BOX)
+    o = zt.Options(64, 64, 2, 400, seed=5)
+    for prec in (N.PRECISION_FP32, N.PRECISION_FP64):
+        base = zt.render(s, o, zt.Tuning(method=MEGA, schedule=1, precision=prec))
+        assert np.isfinite(base).all() and base.max() > 0
+        for tw, th in ((8, 8), (64, 64)):
+            for order in (1, 2):
+                img = zt.render(s, o, zt.Tuning(method=MEGA, schedule=2, tile_order=order, tile_w=tw, tile_h=th,
+                                                precision=prec))
+                assert np.array_equal(base, img), (prec, tw, th, order)
+
+
 def _with_large_leaves(nodes: np.ndarray, cap: int) -> np.ndarray:
     """The host BVH with every subtree of at most `cap` primitives turned into
     one leaf over its (contiguous) primitive range -- the shape of the leaves
